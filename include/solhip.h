@@ -1,0 +1,241 @@
+/*
+ * solhip.h -- C ABI of the MI355X (gfx950) ray-tracing backend for sol-rs.
+ *
+ * Replaces the Vulkan hardware ray-tracing path of sol-rs: the
+ * acceleration-structure and pipeline code of src/ray, the scene upload of
+ * src/scene and the raygen / closest-hit / miss shaders of assets/glsl used by
+ * examples/4-ray-ao.rs and examples/5-pathtrace.rs.  The Rust side binds these
+ * symbols through a thin `extern "C"` crate (sol-hip-sys); the C++ mirror of
+ * sol's ray:: / scene:: API (sol-rs_amd/host/) is written against this header.
+ *
+ * Conventions
+ *   - every entry point returns int32_t: 0 = SOLHIP_OK, negative = error;
+ *     solhip_last_error() returns the message of the last failure on the
+ *     calling thread;
+ *   - handles are opaque uint64_t; every *_create / *_build has a *_destroy;
+ *   - a context is externally synchronized (like a VkQueue); work is enqueued
+ *     on a solhip_stream (a HIP stream; it stands in for the
+ *     vk::CommandBuffer the reference records into) and solhip_stream_sync
+ *     waits for it;
+ *   - device memory is owned by solhip_buffer objects; records passed by
+ *     pointer (geometries, instances, uniforms, tables) are host memory and
+ *     are consumed before the call returns;
+ *   - there is no CPU fallback: without a usable gfx950 device,
+ *     solhip_context_create fails with SOLHIP_ERROR_NO_DEVICE.
+ */
+#ifndef SOLHIP_H
+#define SOLHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SOLHIP_ABI_VERSION 1u
+
+typedef uint64_t solhip_ctx;
+typedef uint64_t solhip_stream;
+typedef uint64_t solhip_buffer;
+typedef uint64_t solhip_blas;
+typedef uint64_t solhip_tlas;
+typedef uint64_t solhip_program;
+
+enum {
+    SOLHIP_OK = 0,
+    SOLHIP_ERROR_INVALID_ARGUMENT = -1,
+    SOLHIP_ERROR_INVALID_HANDLE = -2,
+    SOLHIP_ERROR_OUT_OF_MEMORY = -3,
+    SOLHIP_ERROR_NO_DEVICE = -4,
+    SOLHIP_ERROR_DEVICE = -5,        /* HIP runtime / kernel failure */
+    SOLHIP_ERROR_UNSUPPORTED = -6,
+    SOLHIP_ERROR_OVERFLOW = -7       /* a device-side capacity was exceeded */
+};
+
+/* ---- records with the reference's byte layouts ------------------------- */
+
+/* src/scene/mesh.rs:7-25 ModelVertex (64 B) */
+typedef struct { float pos[4], color[4], normal[4], uv[4]; } solhip_model_vertex;
+
+/* src/scene/mod.rs:19-29 MaterialInfo (48 B) */
+typedef struct {
+    float base_color[4];
+    float emissive[3];
+    float padding0;
+    float metallic, roughness, padding1, padding2;
+} solhip_material;
+
+/* src/ray/mod.rs:16-24 SceneInstance (144 B, column-major mat4s) */
+typedef struct {
+    uint32_t id, texture_offset;
+    float padding[2];
+    float transform[16];
+    float transform_it[16];
+} solhip_scene_instance;
+
+/* src/ray/acceleration.rs:256-306 InstanceDescriptor (64 B):
+ * transform = 3x4 row-major object-to-world, id_mask = custom id:24 | mask:8,
+ * offset_flags = SBT offset:24 | VkGeometryInstanceFlagsKHR:8, and, where the
+ * reference stores the BLAS device address, the solhip_blas handle. */
+typedef struct {
+    float transform[12];
+    uint32_t id_mask;
+    uint32_t offset_flags;
+    uint64_t blas;
+} solhip_instance;
+
+/* examples/4-ray-ao.rs:9-17 / examples/5-pathtrace.rs:9-17 SceneUniforms
+ * (std140, 400 B) */
+typedef struct {
+    float model[16], view[16], view_inverse[16];
+    float projection[16], projection_inverse[16], model_view_projection[16];
+    uint32_t frame[3];
+    uint32_t pad;
+} solhip_scene_uniforms;
+
+/* BLAS geometry: replaces vk::AccelerationStructureGeometryTrianglesDataKHR +
+ * BuildRangeInfo as BLAS::new fills them (src/ray/acceleration.rs:148-208):
+ * positions are R32G32B32_SFLOAT at byte offset 0 of each vertex, vertex i of
+ * a triangle is at vertex_addr + (first_vertex + index) * vertex_stride, and
+ * indices are UINT32 at index_addr + index_byte_offset. */
+typedef struct {
+    uint64_t vertex_addr;        /* device address (solhip_buffer_address) */
+    uint64_t vertex_stride;      /* 64 for ModelVertex */
+    uint32_t first_vertex;
+    uint32_t vertex_count;
+    uint64_t index_addr;         /* device address of a u32 index buffer */
+    uint64_t index_byte_offset;
+    uint32_t index_count;        /* primitive_count = index_count / 3 */
+    uint32_t opaque;             /* GeometryFlagsKHR::OPAQUE; must be 1 */
+} solhip_geometry;
+
+/* ray / hit records of solhip_trace_rays (32 B / 20 B).  Closest hit is the
+ * lexicographic minimum of (t, instance index, primitive index) over all
+ * triangles with t in [tmin, tmax]; on a miss t = -1, inst = prim =
+ * 0xFFFFFFFF.  inst is the TLAS instance index (gl_InstanceID), prim the
+ * triangle index in its BLAS (gl_PrimitiveID), (u, v) the barycentrics of
+ * vertices 1 and 2 (hitAttributeEXT.xy). */
+typedef struct { float ox, oy, oz, tmin, dx, dy, dz, tmax; } solhip_ray;
+typedef struct { float t, u, v; uint32_t inst, prim; } solhip_hit;
+#define SOLHIP_MISS 0xFFFFFFFFu
+
+/* trace flags */
+#define SOLHIP_TRACE_TERMINATE_ON_FIRST_HIT 0x4u  /* gl_RayFlagsTerminateOnFirstHitEXT */
+#define SOLHIP_TRACE_COUNT                  0x100u /* count node/triangle/instance fetches */
+
+/* hit-shader descriptor tables (set 1, bindings 3-6 of the examples):
+ * scnDesc.i[] (SceneInstance), vertices[objId], indices[objId],
+ * materials[gl_InstanceID].  The *_table_addr fields are device addresses of
+ * arrays of uint64 device addresses (one per object / instance). */
+typedef struct {
+    uint64_t instances_addr;        /* solhip_scene_instance[instance_count] */
+    uint32_t instance_count;
+    uint32_t index_size;            /* 8 (the reference's u64 index storage) or 4 */
+    uint64_t vertex_table_addr;     /* u64[objects] -> ModelVertex[] */
+    uint64_t index_table_addr;      /* u64[objects] -> u32/u64 indices */
+    uint64_t material_table_addr;   /* u64[instances] -> MaterialInfo, or 0 */
+    uint32_t object_count;
+    uint32_t reserved;
+} solhip_scene_tables;
+
+/* programs (ray::PipelineInfo shader sets, src/ray/pipeline.rs:5-52) */
+enum {
+    SOLHIP_PROG_AO_GLSL = 1,        /* ao.rgen/rchit/rmiss */
+    SOLHIP_PROG_PATHTRACE_GLSL = 2, /* pathtrace.rgen/rchit/rmiss */
+    SOLHIP_PROG_DEBUG = 3,          /* debug.rgen/rchit/rmiss */
+    SOLHIP_PROG_AO = 4              /* plain ray-traced AO (primary + n_ao rays) */
+};
+
+/* specialization + run configuration; solhip_spec_default gives the GLSL
+ * constants (ao.rgen:42-43, pathtrace.rgen:43-44, pathtrace.rmiss:5). */
+typedef struct {
+    uint32_t enable_sky;      /* specialization constant 0 (ENABLE_SKYLIGHT) */
+    uint32_t spp_per_frame;   /* pathtrace: 8 */
+    uint32_t max_depth;       /* pathtrace: 32 bounces */
+    uint32_t n_ao;            /* SOLHIP_PROG_AO: rays per primary hit (8) */
+    float ao_tmin, ao_tmax;   /* SOLHIP_PROG_AO ray range */
+    uint32_t ao_any_hit;      /* SOLHIP_PROG_AO: occlusion may stop at any hit */
+    uint32_t reserved[9];
+} solhip_spec;
+
+/* per-dispatch images: W*H pixels, row-major, no padding */
+typedef struct {
+    solhip_buffer accum;        /* RGBA32F (accumulation / AO target), read-modify-write */
+    solhip_buffer out_rgba8;    /* RGBA8 output (pathtrace), or 0 */
+    solhip_buffer blue_noise;   /* RGBA8 texture (AO glsl), or 0 */
+    uint32_t blue_w, blue_h;
+    solhip_buffer hits_out;     /* optional: primary hits [W*H], then AO hits [W*H*n_ao] (SOLHIP_PROG_AO) */
+} solhip_images;
+
+typedef struct {
+    uint64_t rays;             /* rays traced (all kinds) */
+    uint64_t node_fetches;     /* with SOLHIP_TRACE_COUNT */
+    uint64_t tri_fetches;
+    uint64_t inst_fetches;
+    uint64_t overflow;         /* traversal stack / queue overflows (must be 0) */
+} solhip_counters;
+
+/* ---- context / streams / errors ---------------------------------------- */
+/* replaces SharedContext::new (src/context.rs:239-370) for the ray path */
+int32_t solhip_context_create(int32_t device, uint32_t flags, solhip_ctx *out);
+int32_t solhip_context_destroy(solhip_ctx ctx);
+uint32_t solhip_abi_version(void);
+int32_t solhip_last_error(char *buf, size_t len);
+int32_t solhip_stream_create(solhip_ctx ctx, solhip_stream *out);
+int32_t solhip_stream_destroy(solhip_ctx ctx, solhip_stream s);
+int32_t solhip_stream_sync(solhip_ctx ctx, solhip_stream s);   /* stream 0 = default */
+
+/* ---- buffers (src/buffer.rs: Buffer::new/from_data/update/get_device_address) */
+int32_t solhip_buffer_create(solhip_ctx ctx, uint64_t bytes, solhip_buffer *out);
+int32_t solhip_buffer_destroy(solhip_ctx ctx, solhip_buffer b);
+int32_t solhip_buffer_upload(solhip_ctx ctx, solhip_buffer b, uint64_t offset, const void *src, uint64_t bytes);
+int32_t solhip_buffer_download(solhip_ctx ctx, solhip_buffer b, uint64_t offset, void *dst, uint64_t bytes);
+int32_t solhip_buffer_fill(solhip_ctx ctx, solhip_stream s, solhip_buffer b, uint32_t value);
+int32_t solhip_buffer_address(solhip_ctx ctx, solhip_buffer b, uint64_t *addr);
+
+/* ---- acceleration structures ------------------------------------------- */
+/* BLAS::new (src/ray/acceleration.rs:136-239): GPU LBVH build, enqueued on s.
+ * Exactly one geometry per BLAS (SceneDescription::from_meshes builds one
+ * BLAS per primitive, src/ray/mod.rs:125-132). */
+int32_t solhip_blas_build(solhip_ctx ctx, solhip_stream s, const solhip_geometry *g, uint32_t n,
+                          uint32_t flags, solhip_blas *out);
+/* rebuild in place from the same (or updated) vertex data */
+int32_t solhip_blas_rebuild(solhip_ctx ctx, solhip_stream s, solhip_blas b, const solhip_geometry *g, uint32_t n);
+int32_t solhip_blas_destroy(solhip_ctx ctx, solhip_blas b);
+/* TLAS::new / TLAS::regenerate (src/ray/acceleration.rs:344-467) */
+int32_t solhip_tlas_build(solhip_ctx ctx, solhip_stream s, const solhip_instance *inst, uint32_t n,
+                          solhip_tlas *out);
+int32_t solhip_tlas_rebuild(solhip_ctx ctx, solhip_stream s, solhip_tlas t, const solhip_instance *inst, uint32_t n);
+int32_t solhip_tlas_destroy(solhip_ctx ctx, solhip_tlas t);
+/* BVH statistics: nodes, triangles, tree depth, bytes */
+typedef struct { uint32_t nodes, prims, depth, leaf_max; uint64_t bytes; } solhip_bvh_info;
+int32_t solhip_blas_info(solhip_ctx ctx, solhip_blas b, solhip_bvh_info *info);
+int32_t solhip_tlas_info(solhip_ctx ctx, solhip_tlas t, solhip_bvh_info *info);
+
+/* ---- tracing ------------------------------------------------------------ */
+/* closest-hit (or first-hit) query for n rays in device buffers; the
+ * behaviour of traceRayEXT with gl_RayFlagsOpaqueEXT and cull mask 0xFF. */
+int32_t solhip_trace_rays(solhip_ctx ctx, solhip_stream s, solhip_tlas t, solhip_buffer rays,
+                          solhip_buffer hits, uint32_t n, uint32_t flags);
+
+/* ray::Pipeline::new + ShaderBindingTable (src/ray/pipeline.rs:61-122,
+ * src/ray/sbt.rs:71-165) */
+void solhip_spec_default(uint32_t kind, solhip_spec *spec);
+int32_t solhip_program_create(solhip_ctx ctx, uint32_t kind, const solhip_spec *spec, solhip_program *out);
+int32_t solhip_program_destroy(solhip_ctx ctx, solhip_program p);
+
+/* ShaderBindingTable::cmd_trace_rays (src/ray/sbt.rs:167-180): one W x H
+ * launch of the program with the examples' bindings (UBO set 0, set 1
+ * tables/images, push constant accum_start_frame). */
+int32_t solhip_trace(solhip_ctx ctx, solhip_stream s, solhip_program p, solhip_tlas t,
+                     const solhip_scene_tables *tables, const solhip_scene_uniforms *uniforms,
+                     int32_t accum_start_frame, const solhip_images *images, uint32_t W, uint32_t H);
+
+/* counters accumulated since the last reset (rays, fetches, overflows) */
+int32_t solhip_stats(solhip_ctx ctx, solhip_counters *out, uint32_t reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SOLHIP_H */

@@ -1,0 +1,720 @@
+// api.hip -- the C ABI of include/solhip.h.
+//
+// Objects live in a handle registry (opaque uint64 ids, type-checked on every
+// call).  Device memory: hipMalloc; uploads/downloads block like
+// Buffer::from_data / Buffer::update of the reference (src/buffer.rs:144-225).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "bvh.h"
+#include "common.h"
+#include "kernels.h"
+
+namespace solhip {
+
+thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+enum class Kind { Context, Stream, Buffer, Blas, Tlas, Program };
+
+struct Obj {
+    Kind kind;
+    uint64_t ctx;
+    virtual ~Obj() = default;
+};
+
+struct Context : Obj {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Counters *counters = nullptr;
+};
+
+struct Stream : Obj {
+    hipStream_t s = nullptr;
+};
+
+struct Buffer : Obj {
+    void *ptr = nullptr;
+    uint64_t bytes = 0;
+};
+
+struct Blas : Obj {
+    DeviceBvh bvh;
+    DevScratch scratch;
+};
+
+struct Tlas : Obj {
+    DeviceBvh bvh;
+    DevScratch scratch;
+    DevBuf inst_dev;   // solhip_instance[]
+    DevBuf refs_dev;   // BlasRef[]
+};
+
+struct Program : Obj {
+    uint32_t kind_id = 0;
+    solhip_spec spec{};
+    DevBuf queue, qcount;
+};
+
+std::mutex g_mu;
+std::unordered_map<uint64_t, Obj *> g_objs;
+uint64_t g_next = 1;
+
+uint64_t reg(Obj *o)
+{
+    std::lock_guard<std::mutex> l(g_mu);
+    const uint64_t h = (g_next++ << 8) | (uint64_t)o->kind;
+    g_objs[h] = o;
+    return h;
+}
+
+template <class T>
+T *get(uint64_t h, Kind k)
+{
+    std::lock_guard<std::mutex> l(g_mu);
+    auto it = g_objs.find(h);
+    if (it == g_objs.end() || it->second->kind != k) return nullptr;
+    return static_cast<T *>(it->second);
+}
+
+Obj *unreg(uint64_t h, Kind k)
+{
+    std::lock_guard<std::mutex> l(g_mu);
+    auto it = g_objs.find(h);
+    if (it == g_objs.end() || it->second->kind != k) return nullptr;
+    Obj *o = it->second;
+    g_objs.erase(it);
+    return o;
+}
+
+int bad_handle(const char *what)
+{
+    set_error(std::string("invalid ") + what + " handle");
+    return SOLHIP_ERROR_INVALID_HANDLE;
+}
+
+int bad_arg(const std::string &what)
+{
+    set_error("invalid argument: " + what);
+    return SOLHIP_ERROR_INVALID_ARGUMENT;
+}
+
+Context *ctx_of(solhip_ctx c)
+{
+    Context *cx = get<Context>(c, Kind::Context);
+    if (cx) (void)hipSetDevice(cx->device);
+    return cx;
+}
+
+hipStream_t stream_of(Context *cx, solhip_stream s, bool &ok)
+{
+    ok = true;
+    if (s == 0) return cx->stream;
+    Stream *st = get<Stream>(s, Kind::Stream);
+    if (!st) {
+        ok = false;
+        return nullptr;
+    }
+    return st->s;
+}
+
+SceneView view_of(const Tlas *t)
+{
+    SceneView v;
+    if (t->bvh.prim_count == 0) {
+        v.tnodes = nullptr;
+        v.insts = nullptr;
+        v.inst_meta = nullptr;
+    } else {
+        v.tnodes = (const WideNode *)t->bvh.nodes.ptr;
+        v.insts = (const InstRec *)t->bvh.leaves.ptr;
+        v.inst_meta = (const uint32_t *)t->bvh.meta.ptr;
+    }
+    return v;
+}
+
+int tlas_build_into(Context *cx, hipStream_t s, Tlas *t, const solhip_instance *inst, uint32_t n)
+{
+    std::vector<BlasRef> refs(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        Blas *b = get<Blas>(inst[i].blas, Kind::Blas);
+        if (!b) {
+            set_error("instance " + std::to_string(i) + " references an invalid BLAS handle");
+            return SOLHIP_ERROR_INVALID_HANDLE;
+        }
+        if (b->bvh.prim_count == 0) {
+            refs[i].nodes = nullptr;
+            refs[i].tris = nullptr;
+        } else {
+            refs[i].nodes = (const WideNode *)b->bvh.nodes.ptr;
+            refs[i].tris = (const TriRec *)b->bvh.leaves.ptr;
+        }
+        refs[i].root_box = (const float *)b->bvh.root_box.ptr;
+        refs[i].pad = 0;
+    }
+    const uint64_t nn = n ? n : 1;
+    if (t->inst_dev.reserve(nn * sizeof(solhip_instance)) || t->refs_dev.reserve(nn * sizeof(BlasRef)))
+        return SOLHIP_ERROR_OUT_OF_MEMORY;
+    if (n) {
+        SOLHIP_HIP_CHECK(hipStreamSynchronize(s));
+        SOLHIP_HIP_CHECK(hipMemcpy(t->inst_dev.ptr, inst, (uint64_t)n * sizeof(solhip_instance), hipMemcpyHostToDevice));
+        SOLHIP_HIP_CHECK(hipMemcpy(t->refs_dev.ptr, refs.data(), (uint64_t)n * sizeof(BlasRef), hipMemcpyHostToDevice));
+    }
+    (void)cx;
+    return build_tlas(s, (const solhip_instance *)t->inst_dev.ptr, (const BlasRef *)t->refs_dev.ptr, n, t->scratch,
+                      t->bvh);
+}
+
+}  // namespace solhip
+
+using namespace solhip;
+
+extern "C" {
+
+uint32_t solhip_abi_version(void) { return SOLHIP_ABI_VERSION; }
+
+int32_t solhip_last_error(char *buf, size_t len)
+{
+    if (!buf || len == 0) return SOLHIP_ERROR_INVALID_ARGUMENT;
+    const size_t n = std::min(len - 1, g_last_error.size());
+    std::memcpy(buf, g_last_error.data(), n);
+    buf[n] = 0;
+    return SOLHIP_OK;
+}
+
+int32_t solhip_context_create(int32_t device, uint32_t flags, solhip_ctx *out)
+{
+    (void)flags;
+    if (!out) return bad_arg("out");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        set_error("no HIP device available (this library runs on gfx950 only; there is no CPU fallback)");
+        return SOLHIP_ERROR_NO_DEVICE;
+    }
+    if (device < 0 || device >= count) return bad_arg("device index");
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        set_error("hipGetDeviceProperties failed");
+        return SOLHIP_ERROR_NO_DEVICE;
+    }
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        set_error(std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950 only");
+        return SOLHIP_ERROR_NO_DEVICE;
+    }
+    SOLHIP_HIP_CHECK(hipSetDevice(device));
+    Context *c = new Context();
+    c->kind = Kind::Context;
+    c->device = device;
+    if (hipStreamCreate(&c->stream) != hipSuccess || hipMalloc(&c->counters, sizeof(Counters)) != hipSuccess) {
+        set_error("context allocation failed");
+        delete c;
+        return SOLHIP_ERROR_DEVICE;
+    }
+    (void)hipMemset(c->counters, 0, sizeof(Counters));
+    (void)hipDeviceSynchronize();
+    const uint64_t h = reg(c);
+    c->ctx = h;
+    *out = h;
+    return SOLHIP_OK;
+}
+
+int32_t solhip_context_destroy(solhip_ctx ctx)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    (void)hipDeviceSynchronize();
+    // release every object created in this context
+    std::vector<uint64_t> mine;
+    {
+        std::lock_guard<std::mutex> l(g_mu);
+        for (auto &kv : g_objs)
+            if (kv.second->ctx == ctx && kv.first != ctx) mine.push_back(kv.first);
+    }
+    for (uint64_t h : mine) {
+        Obj *o = nullptr;
+        {
+            std::lock_guard<std::mutex> l(g_mu);
+            auto it = g_objs.find(h);
+            if (it == g_objs.end()) continue;
+            o = it->second;
+            g_objs.erase(it);
+        }
+        switch (o->kind) {
+        case Kind::Stream: (void)hipStreamDestroy(static_cast<Stream *>(o)->s); break;
+        case Kind::Buffer: (void)hipFree(static_cast<Buffer *>(o)->ptr); break;
+        case Kind::Blas: static_cast<Blas *>(o)->bvh.release(); static_cast<Blas *>(o)->scratch.release(); break;
+        case Kind::Tlas: {
+            Tlas *t = static_cast<Tlas *>(o);
+            t->bvh.release();
+            t->scratch.release();
+            t->inst_dev.release();
+            t->refs_dev.release();
+            break;
+        }
+        case Kind::Program: static_cast<Program *>(o)->queue.release(); static_cast<Program *>(o)->qcount.release(); break;
+        default: break;
+        }
+        delete o;
+    }
+    unreg(ctx, Kind::Context);
+    (void)hipStreamDestroy(c->stream);
+    (void)hipFree(c->counters);
+    delete c;
+    return SOLHIP_OK;
+}
+
+int32_t solhip_stream_create(solhip_ctx ctx, solhip_stream *out)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    if (!out) return bad_arg("out");
+    Stream *s = new Stream();
+    s->kind = Kind::Stream;
+    s->ctx = ctx;
+    if (hipStreamCreate(&s->s) != hipSuccess) {
+        delete s;
+        set_error("hipStreamCreate failed");
+        return SOLHIP_ERROR_DEVICE;
+    }
+    *out = reg(s);
+    return SOLHIP_OK;
+}
+
+int32_t solhip_stream_destroy(solhip_ctx ctx, solhip_stream st)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    Stream *s = static_cast<Stream *>(unreg(st, Kind::Stream));
+    if (!s) return bad_handle("stream");
+    (void)hipStreamSynchronize(s->s);
+    (void)hipStreamDestroy(s->s);
+    delete s;
+    return SOLHIP_OK;
+}
+
+int32_t solhip_stream_sync(solhip_ctx ctx, solhip_stream st)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    bool ok;
+    hipStream_t s = stream_of(c, st, ok);
+    if (!ok) return bad_handle("stream");
+    SOLHIP_HIP_CHECK(hipStreamSynchronize(s));
+    return SOLHIP_OK;
+}
+
+int32_t solhip_buffer_create(solhip_ctx ctx, uint64_t bytes, solhip_buffer *out)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    if (!out) return bad_arg("out");
+    Buffer *b = new Buffer();
+    b->kind = Kind::Buffer;
+    b->ctx = ctx;
+    b->bytes = bytes;
+    if (hipMalloc(&b->ptr, bytes ? bytes : 16) != hipSuccess) {
+        delete b;
+        set_error("hipMalloc of " + std::to_string(bytes) + " bytes failed");
+        return SOLHIP_ERROR_OUT_OF_MEMORY;
+    }
+    *out = reg(b);
+    return SOLHIP_OK;
+}
+
+int32_t solhip_buffer_destroy(solhip_ctx ctx, solhip_buffer bh)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    Buffer *b = static_cast<Buffer *>(unreg(bh, Kind::Buffer));
+    if (!b) return bad_handle("buffer");
+    (void)hipDeviceSynchronize();
+    (void)hipFree(b->ptr);
+    delete b;
+    return SOLHIP_OK;
+}
+
+int32_t solhip_buffer_upload(solhip_ctx ctx, solhip_buffer bh, uint64_t offset, const void *src, uint64_t bytes)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    Buffer *b = get<Buffer>(bh, Kind::Buffer);
+    if (!b) return bad_handle("buffer");
+    if (offset > b->bytes || bytes > b->bytes - offset) return bad_arg("upload range exceeds buffer");
+    if (bytes == 0) return SOLHIP_OK;
+    if (!src) return bad_arg("src");
+    SOLHIP_HIP_CHECK(hipMemcpy((char *)b->ptr + offset, src, bytes, hipMemcpyHostToDevice));
+    return SOLHIP_OK;
+}
+
+int32_t solhip_buffer_download(solhip_ctx ctx, solhip_buffer bh, uint64_t offset, void *dst, uint64_t bytes)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    Buffer *b = get<Buffer>(bh, Kind::Buffer);
+    if (!b) return bad_handle("buffer");
+    if (offset > b->bytes || bytes > b->bytes - offset) return bad_arg("download range exceeds buffer");
+    if (bytes == 0) return SOLHIP_OK;
+    if (!dst) return bad_arg("dst");
+    SOLHIP_HIP_CHECK(hipMemcpy(dst, (const char *)b->ptr + offset, bytes, hipMemcpyDeviceToHost));
+    return SOLHIP_OK;
+}
+
+int32_t solhip_buffer_fill(solhip_ctx ctx, solhip_stream st, solhip_buffer bh, uint32_t value)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    bool ok;
+    hipStream_t s = stream_of(c, st, ok);
+    if (!ok) return bad_handle("stream");
+    Buffer *b = get<Buffer>(bh, Kind::Buffer);
+    if (!b) return bad_handle("buffer");
+    return launch_fill(s, (uint32_t *)b->ptr, b->bytes / 4, value);
+}
+
+int32_t solhip_buffer_address(solhip_ctx ctx, solhip_buffer bh, uint64_t *addr)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    Buffer *b = get<Buffer>(bh, Kind::Buffer);
+    if (!b) return bad_handle("buffer");
+    if (!addr) return bad_arg("addr");
+    *addr = (uint64_t)b->ptr;
+    return SOLHIP_OK;
+}
+
+int32_t solhip_blas_build(solhip_ctx ctx, solhip_stream st, const solhip_geometry *g, uint32_t n, uint32_t flags,
+                          solhip_blas *out)
+{
+    (void)flags;
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    bool ok;
+    hipStream_t s = stream_of(c, st, ok);
+    if (!ok) return bad_handle("stream");
+    if (!g || !out) return bad_arg("geometry / out");
+    if (n != 1) {
+        set_error("exactly one geometry per BLAS is supported (SceneDescription builds one BLAS per primitive)");
+        return SOLHIP_ERROR_UNSUPPORTED;
+    }
+    if (g->index_count && (!g->vertex_addr || !g->index_addr)) return bad_arg("null vertex / index address");
+    if (g->vertex_stride < 12) return bad_arg("vertex_stride < 12");
+    Blas *b = new Blas();
+    b->kind = Kind::Blas;
+    b->ctx = ctx;
+    int rc = build_blas(s, *g, b->scratch, b->bvh);
+    if (rc) {
+        b->bvh.release();
+        b->scratch.release();
+        delete b;
+        return rc;
+    }
+    *out = reg(b);
+    return SOLHIP_OK;
+}
+
+int32_t solhip_blas_rebuild(solhip_ctx ctx, solhip_stream st, solhip_blas bh, const solhip_geometry *g, uint32_t n)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    bool ok;
+    hipStream_t s = stream_of(c, st, ok);
+    if (!ok) return bad_handle("stream");
+    Blas *b = get<Blas>(bh, Kind::Blas);
+    if (!b) return bad_handle("BLAS");
+    if (!g || n != 1) return bad_arg("exactly one geometry");
+    return build_blas(s, *g, b->scratch, b->bvh);
+}
+
+int32_t solhip_blas_destroy(solhip_ctx ctx, solhip_blas bh)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    Blas *b = static_cast<Blas *>(unreg(bh, Kind::Blas));
+    if (!b) return bad_handle("BLAS");
+    (void)hipDeviceSynchronize();
+    b->bvh.release();
+    b->scratch.release();
+    delete b;
+    return SOLHIP_OK;
+}
+
+int32_t solhip_tlas_build(solhip_ctx ctx, solhip_stream st, const solhip_instance *inst, uint32_t n, solhip_tlas *out)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    bool ok;
+    hipStream_t s = stream_of(c, st, ok);
+    if (!ok) return bad_handle("stream");
+    if (!out || (n && !inst)) return bad_arg("instances / out");
+    if (n > 0xFFFFFFu) return bad_arg("more than 2^24 instances");
+    Tlas *t = new Tlas();
+    t->kind = Kind::Tlas;
+    t->ctx = ctx;
+    int rc = tlas_build_into(c, s, t, inst, n);
+    if (rc) {
+        t->bvh.release();
+        t->scratch.release();
+        t->inst_dev.release();
+        t->refs_dev.release();
+        delete t;
+        return rc;
+    }
+    *out = reg(t);
+    return SOLHIP_OK;
+}
+
+int32_t solhip_tlas_rebuild(solhip_ctx ctx, solhip_stream st, solhip_tlas th, const solhip_instance *inst, uint32_t n)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    bool ok;
+    hipStream_t s = stream_of(c, st, ok);
+    if (!ok) return bad_handle("stream");
+    Tlas *t = get<Tlas>(th, Kind::Tlas);
+    if (!t) return bad_handle("TLAS");
+    if (n && !inst) return bad_arg("instances");
+    if (n > 0xFFFFFFu) return bad_arg("more than 2^24 instances");
+    return tlas_build_into(c, s, t, inst, n);
+}
+
+int32_t solhip_tlas_destroy(solhip_ctx ctx, solhip_tlas th)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    Tlas *t = static_cast<Tlas *>(unreg(th, Kind::Tlas));
+    if (!t) return bad_handle("TLAS");
+    (void)hipDeviceSynchronize();
+    t->bvh.release();
+    t->scratch.release();
+    t->inst_dev.release();
+    t->refs_dev.release();
+    delete t;
+    return SOLHIP_OK;
+}
+
+static void fill_info(const DeviceBvh &b, solhip_bvh_info *info)
+{
+    info->nodes = b.node_count;
+    info->prims = b.prim_count;
+    info->depth = b.depth;
+    info->leaf_max = b.leaf_max;
+    info->bytes = (uint64_t)b.node_count * sizeof(WideNode) +
+                  (uint64_t)b.prim_count * (b.kind == 0 ? sizeof(TriRec) : sizeof(InstRec) + 4);
+}
+
+int32_t solhip_blas_info(solhip_ctx ctx, solhip_blas bh, solhip_bvh_info *info)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    Blas *b = get<Blas>(bh, Kind::Blas);
+    if (!b) return bad_handle("BLAS");
+    if (!info) return bad_arg("info");
+    fill_info(b->bvh, info);
+    return SOLHIP_OK;
+}
+
+int32_t solhip_tlas_info(solhip_ctx ctx, solhip_tlas th, solhip_bvh_info *info)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    Tlas *t = get<Tlas>(th, Kind::Tlas);
+    if (!t) return bad_handle("TLAS");
+    if (!info) return bad_arg("info");
+    fill_info(t->bvh, info);
+    return SOLHIP_OK;
+}
+
+int32_t solhip_trace_rays(solhip_ctx ctx, solhip_stream st, solhip_tlas th, solhip_buffer rays, solhip_buffer hits,
+                          uint32_t n, uint32_t flags)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    bool ok;
+    hipStream_t s = stream_of(c, st, ok);
+    if (!ok) return bad_handle("stream");
+    Tlas *t = get<Tlas>(th, Kind::Tlas);
+    if (!t) return bad_handle("TLAS");
+    Buffer *rb = get<Buffer>(rays, Kind::Buffer);
+    Buffer *hb = get<Buffer>(hits, Kind::Buffer);
+    if (!rb || !hb) return bad_handle("buffer");
+    if ((uint64_t)n * sizeof(solhip_ray) > rb->bytes) return bad_arg("ray buffer smaller than n rays");
+    if ((uint64_t)n * sizeof(solhip_hit) > hb->bytes) return bad_arg("hit buffer smaller than n hits");
+    return launch_trace_rays(s, view_of(t), (const solhip_ray *)rb->ptr, (solhip_hit *)hb->ptr, n, flags, c->counters);
+}
+
+void solhip_spec_default(uint32_t kind, solhip_spec *spec)
+{
+    if (!spec) return;
+    std::memset(spec, 0, sizeof(*spec));
+    spec->enable_sky = 0;       // pathtrace.rmiss:5
+    spec->spp_per_frame = kind == SOLHIP_PROG_AO_GLSL ? 4u : 8u;   // ao.rgen:43, pathtrace.rgen:44
+    spec->max_depth = kind == SOLHIP_PROG_AO_GLSL ? 4u : 32u;      // ao.rgen:42, pathtrace.rgen:43
+    spec->n_ao = 8;
+    spec->ao_tmin = 2e-4f;
+    spec->ao_tmax = 0.2f;
+    spec->ao_any_hit = 0;
+}
+
+int32_t solhip_program_create(solhip_ctx ctx, uint32_t kind, const solhip_spec *spec, solhip_program *out)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    if (!out) return bad_arg("out");
+    if (kind < SOLHIP_PROG_AO_GLSL || kind > SOLHIP_PROG_AO) return bad_arg("program kind");
+    Program *p = new Program();
+    p->kind = Kind::Program;
+    p->ctx = ctx;
+    p->kind_id = kind;
+    if (spec)
+        p->spec = *spec;
+    else
+        solhip_spec_default(kind, &p->spec);
+    if (kind == SOLHIP_PROG_AO) {
+        const uint32_t na = p->spec.n_ao;
+        if (na == 0 || na > 64 || (na & (na - 1))) {
+            delete p;
+            set_error("n_ao must be a power of two in [1, 64]");
+            return SOLHIP_ERROR_UNSUPPORTED;
+        }
+    }
+    if (kind == SOLHIP_PROG_PATHTRACE_GLSL && (p->spec.spp_per_frame == 0)) {
+        delete p;
+        return bad_arg("spp_per_frame must be > 0");
+    }
+    *out = reg(p);
+    return SOLHIP_OK;
+}
+
+int32_t solhip_program_destroy(solhip_ctx ctx, solhip_program ph)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    Program *p = static_cast<Program *>(unreg(ph, Kind::Program));
+    if (!p) return bad_handle("program");
+    (void)hipDeviceSynchronize();
+    p->queue.release();
+    p->qcount.release();
+    delete p;
+    return SOLHIP_OK;
+}
+
+int32_t solhip_trace(solhip_ctx ctx, solhip_stream st, solhip_program ph, solhip_tlas th,
+                     const solhip_scene_tables *tables, const solhip_scene_uniforms *u, int32_t accum_start_frame,
+                     const solhip_images *images, uint32_t W, uint32_t H)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    bool ok;
+    hipStream_t s = stream_of(c, st, ok);
+    if (!ok) return bad_handle("stream");
+    Program *p = get<Program>(ph, Kind::Program);
+    if (!p) return bad_handle("program");
+    Tlas *t = get<Tlas>(th, Kind::Tlas);
+    if (!t) return bad_handle("TLAS");
+    if (!u || !images) return bad_arg("uniforms / images");
+    if (W == 0 || H == 0) return SOLHIP_OK;
+    Buffer *acc = get<Buffer>(images->accum, Kind::Buffer);
+    if (!acc) return bad_handle("accum buffer");
+    const uint64_t npix = (uint64_t)W * H;
+    if (acc->bytes < npix * 16) return bad_arg("accum buffer smaller than W*H*16 bytes");
+    const bool needs_tables = p->kind_id != SOLHIP_PROG_DEBUG;
+    if (needs_tables) {
+        if (!tables || !tables->instances_addr || !tables->vertex_table_addr || !tables->index_table_addr)
+            return bad_arg("scene tables");
+        if (tables->index_size != 4 && tables->index_size != 8) return bad_arg("index_size must be 4 or 8");
+        if (p->kind_id == SOLHIP_PROG_PATHTRACE_GLSL && !tables->material_table_addr)
+            return bad_arg("pathtrace needs the material table");
+    }
+    ProgArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.sv = view_of(t);
+    if (tables) {
+        a.tb.instances = (const solhip_scene_instance *)tables->instances_addr;
+        a.tb.vertex_table = (const uint64_t *)tables->vertex_table_addr;
+        a.tb.index_table = (const uint64_t *)tables->index_table_addr;
+        a.tb.material_table = (const uint64_t *)tables->material_table_addr;
+        a.tb.instance_count = tables->instance_count;
+        a.tb.index_size = tables->index_size;
+    }
+    std::memcpy(a.cam.view_inverse, u->view_inverse, 64);
+    std::memcpy(a.cam.projection_inverse, u->projection_inverse, 64);
+    a.W = W;
+    a.H = H;
+    a.frame = u->frame[2];
+    // ao.rgen:78 / pathtrace.rgen:90: 1 / float(frame.z - accum_start_frame + 1), uint arithmetic
+    const uint32_t denom = u->frame[2] - (uint32_t)accum_start_frame + 1u;
+    a.alpha = 1.0f / (float)denom;
+    a.accum = (float4 *)acc->ptr;
+    if (images->out_rgba8) {
+        Buffer *o = get<Buffer>(images->out_rgba8, Kind::Buffer);
+        if (!o) return bad_handle("out_rgba8 buffer");
+        if (o->bytes < npix * 4) return bad_arg("out_rgba8 smaller than W*H*4 bytes");
+        a.out_rgba8 = (uint32_t *)o->ptr;
+    }
+    if (p->kind_id == SOLHIP_PROG_AO_GLSL) {
+        Buffer *bn = get<Buffer>(images->blue_noise, Kind::Buffer);
+        if (!bn) return bad_handle("blue_noise buffer");
+        if (images->blue_w == 0 || images->blue_h == 0 ||
+            bn->bytes < (uint64_t)images->blue_w * images->blue_h * 4)
+            return bad_arg("blue-noise size");
+        a.blue = (const uint32_t *)bn->ptr;
+        a.blue_w = images->blue_w;
+        a.blue_h = images->blue_h;
+    }
+    a.n_ao = p->spec.n_ao;
+    a.ao_tmin = p->spec.ao_tmin;
+    a.ao_tmax = p->spec.ao_tmax;
+    a.ao_any = p->spec.ao_any_hit;
+    a.enable_sky = p->spec.enable_sky;
+    a.spp = p->spec.spp_per_frame;
+    a.max_depth = p->spec.max_depth;
+    a.counters = c->counters;
+    if (images->hits_out) {
+        Buffer *hb = get<Buffer>(images->hits_out, Kind::Buffer);
+        if (!hb) return bad_handle("hits_out buffer");
+        const uint64_t need = p->kind_id == SOLHIP_PROG_AO ? npix * (1 + a.n_ao) * sizeof(solhip_hit) : 0;
+        if (hb->bytes < need) return bad_arg("hits_out too small");
+        a.hits_out = (solhip_hit *)hb->ptr;
+    }
+    switch (p->kind_id) {
+    case SOLHIP_PROG_AO: {
+        if (p->queue.reserve(npix * sizeof(AoEntry)) || p->qcount.reserve(256)) return SOLHIP_ERROR_OUT_OF_MEMORY;
+        a.queue = (AoEntry *)p->queue.ptr;
+        a.qcount = (uint32_t *)p->qcount.ptr;
+        int rc = launch_fill(s, a.qcount, 1, 0u);
+        if (rc) return rc;
+        return launch_ao(s, a);
+    }
+    case SOLHIP_PROG_AO_GLSL: return launch_ao_glsl(s, a);
+    case SOLHIP_PROG_PATHTRACE_GLSL: return launch_pathtrace_glsl(s, a);
+    case SOLHIP_PROG_DEBUG: return launch_debug(s, a);
+    default: return bad_arg("program kind");
+    }
+}
+
+int32_t solhip_stats(solhip_ctx ctx, solhip_counters *out, uint32_t reset)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    if (!out) return bad_arg("out");
+    SOLHIP_HIP_CHECK(hipDeviceSynchronize());
+    Counters h;
+    SOLHIP_HIP_CHECK(hipMemcpy(&h, c->counters, sizeof(h), hipMemcpyDeviceToHost));
+    out->rays = h.rays;
+    out->node_fetches = h.nodes;
+    out->tri_fetches = h.tris;
+    out->inst_fetches = h.insts;
+    out->overflow = h.overflow;
+    if (reset) {
+        SOLHIP_HIP_CHECK(hipMemset(c->counters, 0, sizeof(Counters)));
+        SOLHIP_HIP_CHECK(hipDeviceSynchronize());
+    }
+    return SOLHIP_OK;
+}
+
+}  // extern "C"

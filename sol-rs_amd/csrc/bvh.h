@@ -1,0 +1,61 @@
+// bvh.h -- host-side objects of the BVH builder (device buffers, scratch).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "common.h"
+
+namespace solhip {
+
+// one device allocation that only grows
+struct DevBuf {
+    void *ptr = nullptr;
+    uint64_t bytes = 0;
+    int reserve(uint64_t bytes_needed);
+    void release();
+};
+
+// per-instance reference to a BLAS, read by the TLAS build kernels
+struct BlasRef {
+    const WideNode *nodes;   // nullptr: empty BLAS
+    const TriRec *tris;
+    const float *root_box;   // 6 floats (lo xyz, hi xyz) in device memory
+    uint64_t pad;
+};
+static_assert(sizeof(BlasRef) == 32, "BlasRef layout");
+
+// build scratch, kept by each BLAS / TLAS so that rebuilds allocate nothing
+struct DevScratch {
+    DevBuf boxes, bounds, keys[2], vals[2], hist, left, right, parent, rfirst, flags, nbox, ncount, counters,
+        wsrc, leaf_ids;
+    std::vector<DevBuf> scan_bufs;
+    uint32_t *pinned = nullptr;
+    int ensure(uint32_t n);
+    uint32_t *scan_level(int level, uint32_t count);
+    void release();
+};
+
+struct DeviceBvh {
+    int kind = 0;          // 0 = BLAS (triangles), 1 = TLAS (instances)
+    DevBuf nodes;          // WideNode[node_count]
+    DevBuf leaves;         // TriRec[] or InstRec[]
+    DevBuf meta;           // TLAS: uint32 inst_meta[]
+    DevBuf root_box;       // 6 floats
+    uint32_t node_count = 0, prim_count = 0, depth = 0, leaf_max = 0;
+    void release()
+    {
+        nodes.release();
+        leaves.release();
+        meta.release();
+        root_box.release();
+    }
+};
+
+int build_blas(hipStream_t s, const solhip_geometry &g, DevScratch &sc, DeviceBvh &out);
+int build_tlas(hipStream_t s, const solhip_instance *inst_dev, const BlasRef *refs_dev, uint32_t n, DevScratch &sc,
+               DeviceBvh &out);
+
+}  // namespace solhip

@@ -1,0 +1,645 @@
+// bvh_build.hip -- GPU LBVH build for BLAS (triangles) and TLAS (instances).
+//
+// Replaces the driver's acceleration-structure build behind
+// vkCmdBuildAccelerationStructuresKHR (src/ray/acceleration.rs:44-126 via
+// BLAS::new 136-239 and TLAS::new/regenerate 344-467).
+//
+// Pipeline (all on the stream the caller passes):
+//   1. primitive boxes + centroid bounds   (triangles: 3 vertex fetches at the
+//      caller's stride; instances: 8 transformed corners of the BLAS root box)
+//   2. 30-bit Morton codes of the centroids (10 bits per axis)
+//   3. stable LSD radix sort of (code, index), 8-bit digits: per-tile LDS
+//      histograms, one exclusive scan, per-tile stable scatter whose local
+//      ranks come from wave64 ballots (match on the 8 digit bits)
+//   4. Karras 2012 binary radix tree (index tie-break for equal codes)
+//   5. bottom-up box refit with one agent-scope acq_rel counter per node
+//   6. top-down collapse, one launch per level, to 8-wide nodes: a node
+//      repeatedly opens its largest-area internal child until it has 8
+//      children; children of <= leaf_max primitives become leaves, the others
+//      new nodes; slots are assigned by the octant heuristic; child boxes are
+//      quantised to 8 bits, rounded outward and padded by one step
+//   7. leaf records in node order (48-B triangles / 64-B instances)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "bvh.h"
+#include "bvh_logic.h"
+#include "common.h"
+
+namespace solhip {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kSortItems = 8;                 // keys per thread per tile
+constexpr int kSortTile = kBlock * kSortItems;  // 2048 keys
+
+__device__ __forceinline__ uint32_t f2ord(float f)
+{
+    uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t u)
+{
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+// block-wide min/max of centroids, then one atomic per block and component
+__device__ void reduce_bounds(float c[3], bool valid, uint32_t *bounds)
+{
+    __shared__ uint32_t smin[3][kBlock / 64], smax[3][kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        uint32_t mn = valid ? f2ord(c[a]) : 0xFFFFFFFFu;
+        uint32_t mx = valid ? f2ord(c[a]) : 0u;
+        for (int off = 32; off > 0; off >>= 1) {
+            mn = min(mn, (uint32_t)__shfl_xor((int)mn, off));
+            mx = max(mx, (uint32_t)__shfl_xor((int)mx, off));
+        }
+        if (lane == 0) {
+            smin[a][wave] = mn;
+            smax[a][wave] = mx;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int a = threadIdx.x;
+        uint32_t mn = 0xFFFFFFFFu, mx = 0u;
+        for (int w = 0; w < kBlock / 64; ++w) {
+            mn = min(mn, smin[a][w]);
+            mx = max(mx, smax[a][w]);
+        }
+        atomicMin(&bounds[a], mn);
+        atomicMax(&bounds[3 + a], mx);
+    }
+}
+
+// vertex of index value `iv`; out-of-range index values are clamped to the
+// last vertex so that malformed input cannot read outside the buffer
+__device__ __forceinline__ const float *vtx(const uint8_t *vbase, uint64_t stride, uint32_t first_vertex,
+                                            uint32_t vertex_count, uint32_t iv)
+{
+    const uint32_t r = iv < vertex_count ? iv : (vertex_count ? vertex_count - 1 : 0);
+    return reinterpret_cast<const float *>(vbase + (uint64_t)(first_vertex + r) * stride);
+}
+
+__global__ void __launch_bounds__(kBlock) k_tri_boxes(const uint8_t *vbase, uint64_t stride, uint32_t first_vertex,
+                                                      uint32_t vertex_count, const uint32_t *idx, uint32_t n,
+                                                      float4 *boxes, uint32_t *bounds)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    float c[3] = {0.0f, 0.0f, 0.0f};
+    const bool valid = i < n;
+    if (valid) {
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int k = 0; k < 3; ++k) {
+            const float *p = vtx(vbase, stride, first_vertex, vertex_count, idx[3ull * i + k]);
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = fminf(lo[a], p[a]);
+                hi[a] = fmaxf(hi[a], p[a]);
+            }
+        }
+        boxes[2ull * i] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+        boxes[2ull * i + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+        for (int a = 0; a < 3; ++a) c[a] = 0.5f * lo[a] + 0.5f * hi[a];
+    }
+    reduce_bounds(c, valid, bounds);
+}
+
+// instance world boxes: the 8 corners of the BLAS root box through the 3x4
+// object-to-world transform, padded relatively
+__global__ void __launch_bounds__(kBlock) k_inst_boxes(const solhip_instance *inst, const BlasRef *refs, uint32_t n,
+                                                       float4 *boxes, uint32_t *bounds)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    float c[3] = {0.0f, 0.0f, 0.0f};
+    const bool valid = i < n;
+    if (valid) {
+        float lo[3], hi[3];
+        instance_box(inst[i].transform, refs[i].nodes != nullptr, refs[i].root_box, lo, hi);
+        boxes[2ull * i] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+        boxes[2ull * i + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+        for (int a = 0; a < 3; ++a) c[a] = 0.5f * lo[a] + 0.5f * hi[a];
+    }
+    reduce_bounds(c, valid, bounds);
+}
+
+__global__ void __launch_bounds__(kBlock) k_morton(const float4 *boxes, uint32_t n, const uint32_t *bounds,
+                                                   uint32_t *keys, uint32_t *vals)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 lo = boxes[2ull * i], hi = boxes[2ull * i + 1];
+    const float l3[3] = {lo.x, lo.y, lo.z}, h3[3] = {hi.x, hi.y, hi.z};
+    const float blo[3] = {ord2f(bounds[0]), ord2f(bounds[1]), ord2f(bounds[2])};
+    const float bhi[3] = {ord2f(bounds[3]), ord2f(bounds[4]), ord2f(bounds[5])};
+    keys[i] = morton30(l3, h3, blo, bhi);
+    vals[i] = i;
+}
+
+// ---- radix sort ---------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_hist(const uint32_t *keys, uint32_t n, int shift, uint32_t *hist,
+                                                 uint32_t ntiles)
+{
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const uint64_t i = base + (uint64_t)r * kBlock + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xFFu], 1u);
+    }
+    __syncthreads();
+    hist[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// scan of up to 1024 * kScanItems values per block; block sums out
+constexpr int kScanItems = 4;
+__global__ void __launch_bounds__(kBlock) k_scan_block(uint32_t *data, uint32_t n, uint32_t *block_sums)
+{
+    __shared__ uint32_t s[kBlock];
+    const uint64_t base = (uint64_t)blockIdx.x * kBlock * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint64_t i = base + (uint64_t)threadIdx.x * kScanItems + k;
+        v[k] = i < n ? data[i] : 0u;
+        sum += v[k];
+    }
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        const uint32_t t = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t run = s[threadIdx.x] - sum;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint64_t i = base + (uint64_t)threadIdx.x * kScanItems + k;
+        if (i < n) data[i] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == kBlock - 1 && block_sums) block_sums[blockIdx.x] = s[kBlock - 1];
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_add(uint32_t *data, uint32_t n, const uint32_t *block_offs)
+{
+    const uint64_t base = (uint64_t)blockIdx.x * kBlock * kScanItems;
+    const uint32_t add = block_offs[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint64_t i = base + (uint64_t)threadIdx.x * kScanItems + k;
+        if (i < n) data[i] += add;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_scatter(const uint32_t *keys_in, const uint32_t *vals_in, uint32_t n,
+                                                    int shift, const uint32_t *offs, uint32_t ntiles,
+                                                    uint32_t *keys_out, uint32_t *vals_out, uint32_t *err)
+{
+    __shared__ uint32_t base_off[256];   // global offset of this tile's digits, advanced per round
+    __shared__ uint32_t wave_cnt[kBlock / 64][256];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    base_off[threadIdx.x] = offs[(uint64_t)threadIdx.x * ntiles + blockIdx.x];
+    const uint64_t tile = (uint64_t)blockIdx.x * kSortTile;
+    const uint64_t lanes_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int r = 0; r < kSortItems; ++r) {
+        for (int w = 0; w < kBlock / 64; ++w) wave_cnt[w][threadIdx.x] = 0;
+        __syncthreads();
+        const uint64_t i = tile + (uint64_t)r * kBlock + threadIdx.x;
+        const bool valid = i < n;
+        const uint32_t key = valid ? keys_in[i] : 0u;
+        const uint32_t val = valid ? vals_in[i] : 0u;
+        const uint32_t digit = (key >> shift) & 0xFFu;
+        // lanes of this wave holding the same digit (8 ballots)
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const uint64_t m = __ballot((digit >> b) & 1u);
+            peers &= ((digit >> b) & 1u) ? m : ~m;
+        }
+        const uint32_t rank_in_wave = __popcll(peers & lanes_below);
+        if (valid && rank_in_wave == 0) wave_cnt[wave][digit] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = base_off[digit] + rank_in_wave;
+            for (int w = 0; w < wave; ++w) pos += wave_cnt[w][digit];
+            if (pos < n) {
+                keys_out[pos] = key;
+                vals_out[pos] = val;
+            } else {
+                atomicAdd(err, 1u);
+            }
+        }
+        __syncthreads();
+        uint32_t tot = 0;
+        for (int w = 0; w < kBlock / 64; ++w) tot += wave_cnt[w][threadIdx.x];
+        base_off[threadIdx.x] += tot;
+        __syncthreads();
+    }
+}
+
+// ---- Karras 2012 radix tree ---------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_karras(const uint32_t *keys, int n, uint32_t *left, uint32_t *right,
+                                                   uint32_t *parent, uint32_t *rfirst)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n - 1) return;
+    karras_node(keys, n, i, left, right, parent, rfirst);
+    if (i == 0) parent[0] = 0xFFFFFFFFu;
+}
+
+// bottom-up boxes; nbox[2*node] = lo, [2*node+1] = hi; ncount = primitives
+__global__ void __launch_bounds__(kBlock) k_refit(const float4 *pbox, const uint32_t *vals, int n,
+                                                  const uint32_t *left, const uint32_t *right,
+                                                  const uint32_t *parent, uint32_t *flags, float4 *nbox,
+                                                  uint32_t *ncount, uint32_t *err)
+{
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    uint32_t node = (uint32_t)(n - 1 + k);
+    const uint32_t p = vals[k];
+    float4 lo = pbox[2ull * p], hi = pbox[2ull * p + 1];
+    nbox[2ull * node] = lo;
+    nbox[2ull * node + 1] = hi;
+    ncount[node] = 1;
+    if (n == 1) return;
+    while (true) {
+        const uint32_t par = parent[node];
+        if (par == 0xFFFFFFFFu) break;
+        if (par >= (uint32_t)(n - 1)) {
+            atomicAdd(err, 1u);
+            break;
+        }
+        const uint32_t old = __hip_atomic_fetch_add(&flags[par], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0) break;
+        const uint32_t l = left[par], r = right[par];
+        if (l >= 2u * (uint32_t)n - 1u || r >= 2u * (uint32_t)n - 1u) {
+            atomicAdd(err, 1u);
+            break;
+        }
+        const float4 al = nbox[2ull * l], ah = nbox[2ull * l + 1];
+        const float4 bl = nbox[2ull * r], bh = nbox[2ull * r + 1];
+        lo = make_float4(fminf(al.x, bl.x), fminf(al.y, bl.y), fminf(al.z, bl.z), 0.0f);
+        hi = make_float4(fmaxf(ah.x, bh.x), fmaxf(ah.y, bh.y), fmaxf(ah.z, bh.z), 0.0f);
+        nbox[2ull * par] = lo;
+        nbox[2ull * par + 1] = hi;
+        ncount[par] = ncount[l] + ncount[r];
+        node = par;
+    }
+}
+
+// ---- collapse to 8-wide quantised nodes ---------------------------------
+struct DevAlloc {
+    uint32_t *counters;
+    __device__ uint32_t operator()(int which, uint32_t amount) const { return atomicAdd(&counters[which], amount); }
+};
+
+__global__ void __launch_bounds__(64) k_collapse(CollapseArgs A)
+{
+    const uint32_t w = A.begin + blockIdx.x * 64 + threadIdx.x;
+    if (w >= A.end) return;
+    if (collapse_node(A, w, DevAlloc{A.counters})) atomicAdd(&A.counters[2], 1u);
+}
+
+__global__ void __launch_bounds__(kBlock) k_write_tris(const uint32_t *leaf_ids, uint32_t n, const uint8_t *vbase,
+                                                       uint64_t stride, uint32_t first_vertex, uint32_t vertex_count,
+                                                       const uint32_t *idx, TriRec *out)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t p = leaf_ids[i];
+    TriRec r;
+    for (int k = 0; k < 3; ++k) {
+        const float *q = vtx(vbase, stride, first_vertex, vertex_count, idx[3ull * p + k]);
+        r.v[3 * k] = q[0];
+        r.v[3 * k + 1] = q[1];
+        r.v[3 * k + 2] = q[2];
+    }
+    r.prim = p;
+    r.pad[0] = r.pad[1] = 0;
+    out[i] = r;
+}
+
+__global__ void __launch_bounds__(kBlock) k_write_insts(const uint32_t *leaf_ids, uint32_t n,
+                                                        const solhip_instance *inst, const BlasRef *refs,
+                                                        InstRec *out, uint32_t *meta)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t p = leaf_ids[i];
+    InstRec r;
+    inverse3x4(inst[p].transform, r.inv);
+    r.nodes = refs[p].nodes;
+    r.tris = refs[p].tris;
+    out[i] = r;
+    meta[i] = (p & 0xFFFFFFu) | (inst[p].id_mask & 0xFF000000u);
+}
+
+// structural check of the collapsed tree before any traversal touches it
+__global__ void __launch_bounds__(kBlock) k_validate(const WideNode *wn, uint32_t nodes, uint32_t prims, uint32_t *bad)
+{
+    const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
+    if (w >= nodes) return;
+    if (!node_valid(wn[w], w, nodes, prims)) atomicAdd(bad, 1u);
+}
+
+__global__ void k_root_box(const float4 *nbox, uint32_t root, float *out)
+{
+    if (threadIdx.x == 0) {
+        const float4 l = nbox[2ull * root], h = nbox[2ull * root + 1];
+        out[0] = l.x;
+        out[1] = l.y;
+        out[2] = l.z;
+        out[3] = h.x;
+        out[4] = h.y;
+        out[5] = h.z;
+    }
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v)
+{
+    if (threadIdx.x < n) p[threadIdx.x] = v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill_u32_n(uint32_t *p, uint32_t n, uint32_t v)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+int exclusive_scan(hipStream_t s, uint32_t *data, uint32_t n, DevScratch &scratch, int level)
+{
+    const uint32_t per_block = kBlock * kScanItems;
+    const uint32_t nb = cdiv(n, per_block);
+    if (nb <= 1) {
+        hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(kBlock), 0, s, data, n, (uint32_t *)nullptr);
+        return 0;
+    }
+    uint32_t *sums = scratch.scan_level(level, nb);
+    if (!sums) return SOLHIP_ERROR_OUT_OF_MEMORY;
+    hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(kBlock), 0, s, data, n, sums);
+    int rc = exclusive_scan(s, sums, nb, scratch, level + 1);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kBlock), 0, s, data, n, (const uint32_t *)sums);
+    return 0;
+}
+
+}  // namespace
+
+uint32_t *DevScratch::scan_level(int level, uint32_t count)
+{
+    if (level >= (int)scan_bufs.size()) scan_bufs.resize(level + 1);
+    DevBuf &b = scan_bufs[level];
+    if (b.bytes < (uint64_t)count * 4) {
+        if (b.reserve((uint64_t)count * 4) != 0) return nullptr;
+    }
+    return (uint32_t *)b.ptr;
+}
+
+int DevBuf::reserve(uint64_t bytes_needed)
+{
+    if (bytes >= bytes_needed && ptr) return 0;
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+    uint64_t want = std::max<uint64_t>(bytes_needed, 256);
+    if (hipMalloc(&ptr, want) != hipSuccess) {
+        ptr = nullptr;
+        set_error("hipMalloc failed in BVH build (" + std::to_string(want) + " bytes)");
+        return SOLHIP_ERROR_OUT_OF_MEMORY;
+    }
+    bytes = want;
+    return 0;
+}
+
+void DevBuf::release()
+{
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+// Generic build over primitive boxes already in scratch.boxes (2 float4 per
+// primitive) with centroid bounds in scratch.bounds.
+static int build_common(hipStream_t s, uint32_t n, uint32_t leaf_max, DevScratch &sc, DeviceBvh &out)
+{
+    const uint32_t g = cdiv(n, kBlock);
+    // counters: [0] wide nodes allocated, [1] leaf records allocated,
+    // [2] collapse range errors, [3] sort / refit range errors, [4] invalid nodes
+    uint32_t *counters = (uint32_t *)sc.counters.ptr;
+    uint32_t *pinned = sc.pinned;
+    pinned[8] = 1u;
+    pinned[9] = pinned[10] = pinned[11] = pinned[12] = 0u;
+    SOLHIP_HIP_CHECK(hipMemcpyAsync(counters, pinned + 8, 20, hipMemcpyHostToDevice, s));
+    // Morton codes
+    hipLaunchKernelGGL(k_morton, dim3(g), dim3(kBlock), 0, s, (const float4 *)sc.boxes.ptr, n,
+                       (const uint32_t *)sc.bounds.ptr, (uint32_t *)sc.keys[0].ptr, (uint32_t *)sc.vals[0].ptr);
+    // radix sort, 4 passes of 8 bits (codes have 30 bits)
+    const uint32_t ntiles = cdiv(n, kSortTile);
+    int cur = 0;
+    for (int pass = 0; pass < 4; ++pass) {
+        const int shift = pass * 8;
+        hipLaunchKernelGGL(k_hist, dim3(ntiles), dim3(kBlock), 0, s, (const uint32_t *)sc.keys[cur].ptr, n, shift,
+                           (uint32_t *)sc.hist.ptr, ntiles);
+        int rc = exclusive_scan(s, (uint32_t *)sc.hist.ptr, 256u * ntiles, sc, 0);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_scatter, dim3(ntiles), dim3(kBlock), 0, s, (const uint32_t *)sc.keys[cur].ptr,
+                           (const uint32_t *)sc.vals[cur].ptr, n, shift, (const uint32_t *)sc.hist.ptr, ntiles,
+                           (uint32_t *)sc.keys[cur ^ 1].ptr, (uint32_t *)sc.vals[cur ^ 1].ptr, counters + 3);
+        cur ^= 1;
+    }
+    const uint32_t *keys = (const uint32_t *)sc.keys[cur].ptr;
+    const uint32_t *vals = (const uint32_t *)sc.vals[cur].ptr;
+    uint32_t *left = (uint32_t *)sc.left.ptr, *right = (uint32_t *)sc.right.ptr;
+    uint32_t *parent = (uint32_t *)sc.parent.ptr, *rfirst = (uint32_t *)sc.rfirst.ptr;
+    uint32_t *flags = (uint32_t *)sc.flags.ptr;
+    float4 *nbox = (float4 *)sc.nbox.ptr;
+    uint32_t *ncount = (uint32_t *)sc.ncount.ptr;
+    if (n > 1) {
+        hipLaunchKernelGGL(k_karras, dim3(cdiv(n - 1, kBlock)), dim3(kBlock), 0, s, keys, (int)n, left, right, parent,
+                           rfirst);
+        hipLaunchKernelGGL(k_fill_u32_n, dim3(cdiv(n - 1, kBlock)), dim3(kBlock), 0, s, flags, n - 1, 0u);
+    }
+    hipLaunchKernelGGL(k_refit, dim3(g), dim3(kBlock), 0, s, (const float4 *)sc.boxes.ptr, vals, (int)n, left, right,
+                       parent, flags, nbox, ncount, counters + 3);
+    // binary root: internal node 0, or leaf 0 (node index n-1 = 0) when n == 1
+    const uint32_t root = 0u;
+    hipLaunchKernelGGL(k_root_box, dim3(1), dim3(64), 0, s, (const float4 *)nbox, root, (float *)out.root_box.ptr);
+    // collapse, level by level
+    WideNode *wn = (WideNode *)out.nodes.ptr;
+    uint32_t *wsrc = (uint32_t *)sc.wsrc.ptr;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, wsrc, 1u, root);
+    uint32_t begin = 0, end = 1, depth = 0;
+    while (begin < end) {
+        CollapseArgs a;
+        a.n = (int)n;
+        a.leaf_max = leaf_max;
+        a.node_cap = n;
+        a.left = left;
+        a.right = right;
+        a.rfirst = rfirst;
+        a.ncount = ncount;
+        a.nbox = nbox;
+        a.vals = vals;
+        a.wnodes = wn;
+        a.wsrc = wsrc;
+        a.counters = counters;
+        a.leaf_ids = (uint32_t *)sc.leaf_ids.ptr;
+        a.begin = begin;
+        a.end = end;
+        hipLaunchKernelGGL(k_collapse, dim3(cdiv(end - begin, 64)), dim3(64), 0, s, a);
+        SOLHIP_HIP_CHECK(hipMemcpyAsync(pinned, counters, 16, hipMemcpyDeviceToHost, s));
+        SOLHIP_HIP_CHECK(hipStreamSynchronize(s));
+        if (pinned[2] || pinned[3]) {
+            set_error("BVH build range check failed (collapse " + std::to_string(pinned[2]) + ", sort/refit " +
+                      std::to_string(pinned[3]) + ")");
+            return SOLHIP_ERROR_DEVICE;
+        }
+        begin = end;
+        end = pinned[0];
+        ++depth;
+        if (end > n || depth > 4096) {
+            set_error("BVH collapse did not converge");
+            return SOLHIP_ERROR_DEVICE;
+        }
+    }
+    out.node_count = end;
+    out.depth = depth;
+    out.prim_count = n;
+    if (pinned[1] != n) {
+        set_error("BVH collapse placed " + std::to_string(pinned[1]) + " of " + std::to_string(n) + " primitives");
+        return SOLHIP_ERROR_DEVICE;
+    }
+    hipLaunchKernelGGL(k_validate, dim3(cdiv(end, kBlock)), dim3(kBlock), 0, s, (const WideNode *)wn, end, n,
+                       counters + 4);
+    SOLHIP_HIP_CHECK(hipMemcpyAsync(pinned + 4, counters + 4, 4, hipMemcpyDeviceToHost, s));
+    SOLHIP_HIP_CHECK(hipStreamSynchronize(s));
+    if (pinned[4] != 0) {
+        set_error("BVH validation failed on " + std::to_string(pinned[4]) + " nodes");
+        return SOLHIP_ERROR_DEVICE;
+    }
+    return 0;
+}
+
+int DevScratch::ensure(uint32_t n)
+{
+    const uint64_t N = std::max<uint32_t>(n, 1);
+    const uint64_t ntiles = (N + kSortTile - 1) / kSortTile;
+    int rc = 0;
+    rc |= boxes.reserve(N * 32);
+    rc |= bounds.reserve(64);
+    rc |= keys[0].reserve(N * 4);
+    rc |= keys[1].reserve(N * 4);
+    rc |= vals[0].reserve(N * 4);
+    rc |= vals[1].reserve(N * 4);
+    rc |= hist.reserve(256 * ntiles * 4);
+    rc |= left.reserve(N * 4);
+    rc |= right.reserve(N * 4);
+    rc |= parent.reserve(2 * N * 4);
+    rc |= rfirst.reserve(N * 4);
+    rc |= flags.reserve(N * 4);
+    rc |= nbox.reserve(2 * N * 32);
+    rc |= ncount.reserve(2 * N * 4);
+    rc |= counters.reserve(256);
+    rc |= wsrc.reserve(N * 4);
+    rc |= leaf_ids.reserve(N * 4);
+    if (!pinned) {
+        if (hipHostMalloc((void **)&pinned, 256, hipHostMallocDefault) != hipSuccess) {
+            pinned = nullptr;
+            set_error("hipHostMalloc failed");
+            return SOLHIP_ERROR_OUT_OF_MEMORY;
+        }
+    }
+    return rc ? SOLHIP_ERROR_OUT_OF_MEMORY : 0;
+}
+
+void DevScratch::release()
+{
+    for (DevBuf *b : {&boxes, &bounds, &keys[0], &keys[1], &vals[0], &vals[1], &hist, &left, &right, &parent, &rfirst,
+                      &flags, &nbox, &ncount, &counters, &wsrc, &leaf_ids})
+        b->release();
+    for (auto &b : scan_bufs) b.release();
+    scan_bufs.clear();
+    if (pinned) (void)hipHostFree(pinned);
+    pinned = nullptr;
+}
+
+static const uint32_t kOrdInit[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+
+int build_blas(hipStream_t s, const solhip_geometry &g, DevScratch &sc, DeviceBvh &out)
+{
+    if (g.index_count % 3 != 0) {
+        set_error("index_count must be a multiple of 3");
+        return SOLHIP_ERROR_INVALID_ARGUMENT;
+    }
+    const uint32_t n = g.index_count / 3;
+    int rc = sc.ensure(n);
+    if (rc) return rc;
+    const uint64_t nn = std::max<uint32_t>(n, 1);
+    rc = out.nodes.reserve(nn * sizeof(WideNode));
+    rc |= out.leaves.reserve(nn * sizeof(TriRec));
+    rc |= out.root_box.reserve(64);
+    if (rc) return SOLHIP_ERROR_OUT_OF_MEMORY;
+    out.kind = 0;
+    out.leaf_max = kLeafMaxTri;
+    if (n == 0) {
+        out.node_count = 0;
+        out.prim_count = 0;
+        out.depth = 0;
+        return 0;
+    }
+    SOLHIP_HIP_CHECK(hipMemcpyAsync(sc.bounds.ptr, kOrdInit, 24, hipMemcpyHostToDevice, s));
+    const uint8_t *vbase = reinterpret_cast<const uint8_t *>(g.vertex_addr);
+    const uint32_t *idx = reinterpret_cast<const uint32_t *>(g.index_addr + g.index_byte_offset);
+    hipLaunchKernelGGL(k_tri_boxes, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, vbase, g.vertex_stride, g.first_vertex,
+                       g.vertex_count, idx, n, (float4 *)sc.boxes.ptr, (uint32_t *)sc.bounds.ptr);
+    rc = build_common(s, n, kLeafMaxTri, sc, out);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_write_tris, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, (const uint32_t *)sc.leaf_ids.ptr, n,
+                       vbase, g.vertex_stride, g.first_vertex, g.vertex_count, idx, (TriRec *)out.leaves.ptr);
+    SOLHIP_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int build_tlas(hipStream_t s, const solhip_instance *inst_dev, const BlasRef *refs_dev, uint32_t n, DevScratch &sc,
+               DeviceBvh &out)
+{
+    int rc = sc.ensure(n);
+    if (rc) return rc;
+    const uint64_t nn = std::max<uint32_t>(n, 1);
+    rc = out.nodes.reserve(nn * sizeof(WideNode));
+    rc |= out.leaves.reserve(nn * sizeof(InstRec));
+    rc |= out.meta.reserve(nn * 4);
+    rc |= out.root_box.reserve(64);
+    if (rc) return SOLHIP_ERROR_OUT_OF_MEMORY;
+    out.kind = 1;
+    out.leaf_max = kLeafMaxInst;
+    if (n == 0) {
+        out.node_count = 0;
+        out.prim_count = 0;
+        out.depth = 0;
+        return 0;
+    }
+    SOLHIP_HIP_CHECK(hipMemcpyAsync(sc.bounds.ptr, kOrdInit, 24, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_inst_boxes, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, inst_dev, refs_dev, n,
+                       (float4 *)sc.boxes.ptr, (uint32_t *)sc.bounds.ptr);
+    rc = build_common(s, n, kLeafMaxInst, sc, out);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_write_insts, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, (const uint32_t *)sc.leaf_ids.ptr, n,
+                       inst_dev, refs_dev, (InstRec *)out.leaves.ptr, (uint32_t *)out.meta.ptr);
+    SOLHIP_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace solhip

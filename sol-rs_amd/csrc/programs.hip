@@ -1,0 +1,477 @@
+// programs.hip -- the ray-tracing programs as kernels.
+//
+//   SOLHIP_PROG_AO            plain ray-traced AO as a wavefront:
+//                             k_ao_primary  generate + extend + shade of the
+//                                           primary rays; hits are compacted
+//                                           into the AO queue with one atomic
+//                                           per wave (ballot + mbcnt)
+//                             k_ao_rays     n_ao rays per queue entry, the
+//                                           lanes of one pixel reduce their
+//                                           occlusion with a ballot
+//   SOLHIP_PROG_AO_GLSL       ao.rgen / ao.rchit / ao.rmiss
+//   SOLHIP_PROG_PATHTRACE_GLSL pathtrace.rgen / pathtrace.rchit / pathtrace.rmiss
+//   SOLHIP_PROG_DEBUG         debug.rgen / debug.rchit / debug.rmiss
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "traverse.h"
+
+namespace solhip {
+
+constexpr int kProgBlock = 64;
+
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
+{
+    const uint32_t x = b & 7u, q = nb >> 3, r = nb & 7u;
+    return x * q + min(x, r) + (b >> 3);
+}
+
+// pixel of this lane: 8x8 tiles, tiles in row-major order, XCD-contiguous
+__device__ __forceinline__ bool tile_pixel(uint32_t W, uint32_t H, uint32_t &x, uint32_t &y)
+{
+    const uint32_t tiles_x = (W + 7) / 8;
+    const uint32_t t = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t tx = t % tiles_x, ty = t / tiles_x;
+    x = tx * 8 + (threadIdx.x & 7);
+    y = ty * 8 + (threadIdx.x >> 3);
+    return x < W && y < H;
+}
+
+struct Surf {
+    V3 pos, nrm, col;
+};
+
+// ao.rchit:56-77 / pathtrace.rchit:59-88: world-space hit point, normal
+// (transform_it, normalised) and interpolated vertex colour
+__device__ __forceinline__ Surf fetch_surface(const DevTables &tb, uint32_t inst, uint32_t prim, float u, float v)
+{
+    const solhip_scene_instance *si = tb.instances + inst;
+    const uint32_t obj = si->id;
+    uint32_t i0, i1, i2;
+    if (tb.index_size == 8) {
+        const uint64_t *ib = reinterpret_cast<const uint64_t *>(tb.index_table[obj]) + 3ull * prim;
+        i0 = (uint32_t)ib[0];
+        i1 = (uint32_t)ib[1];
+        i2 = (uint32_t)ib[2];
+    } else {
+        const uint32_t *ib = reinterpret_cast<const uint32_t *>(tb.index_table[obj]) + 3ull * prim;
+        i0 = ib[0];
+        i1 = ib[1];
+        i2 = ib[2];
+    }
+    const float4 *vb = reinterpret_cast<const float4 *>(tb.vertex_table[obj]);
+    const float4 p0 = vb[4ull * i0], c0 = vb[4ull * i0 + 1], n0 = vb[4ull * i0 + 2];
+    const float4 p1 = vb[4ull * i1], c1 = vb[4ull * i1 + 1], n1 = vb[4ull * i1 + 2];
+    const float4 p2 = vb[4ull * i2], c2 = vb[4ull * i2 + 1], n2 = vb[4ull * i2 + 2];
+    const float bx = 1.0f - u - v, by = u, bz = v;
+    const V3 n = v3(n0.x * bx + n1.x * by + n2.x * bz, n0.y * bx + n1.y * by + n2.y * bz,
+                    n0.z * bx + n1.z * by + n2.z * bz);
+    const V3 p = v3(p0.x * bx + p1.x * by + p2.x * bz, p0.y * bx + p1.y * by + p2.y * bz,
+                    p0.z * bx + p1.z * by + p2.z * bz);
+    Surf s;
+    s.col = v3(c0.x * bx + c1.x * by + c2.x * bz, c0.y * bx + c1.y * by + c2.y * bz,
+               c0.z * bx + c1.z * by + c2.z * bz);
+    float nw[4], pw[4];
+    mat4_mul(si->transform_it, n.x, n.y, n.z, 0.0f, nw);
+    s.nrm = normalize(v3(nw[0], nw[1], nw[2]));
+    mat4_mul(si->transform, p.x, p.y, p.z, 1.0f, pw);
+    s.pos = v3(pw[0], pw[1], pw[2]);
+    return s;
+}
+
+__device__ __forceinline__ void store_hit(solhip_hit *dst, const Hit &h)
+{
+    solhip_hit o;
+    o.t = h.t;
+    o.u = h.u;
+    o.v = h.v;
+    o.inst = h.inst;
+    o.prim = h.prim;
+    *dst = o;
+}
+
+__device__ __forceinline__ void count_rays(Counters *c, uint32_t rays_this_lane, bool overflow)
+{
+    unsigned long long r = rays_this_lane;
+    for (int off = 32; off > 0; off >>= 1) r += __shfl_xor(r, off);
+    const uint64_t ov = __ballot(overflow);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&c->rays, r);
+        if (ov) atomicAdd(&c->overflow, (unsigned long long)__popcll(ov));
+    }
+}
+
+__device__ __forceinline__ float length3(V3 a) { return sqrtf(dot(a, a)); }
+
+// ---------------------------------------------------------------------------
+// plain ray-traced AO (wavefront)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kProgBlock) k_ao_primary(ProgArgs A)
+{
+    __shared__ uint2 stack[kLdsStack * kProgBlock];
+    uint32_t x, y;
+    const bool active = tile_pixel(A.W, A.H, x, y);
+    TravCounts tc{0, 0, 0, false};
+    bool hit = false;
+    AoEntry e;
+    if (active) {
+        const uint32_t pix = y * A.W + x;
+        uint32_t rng = tea(x + y * A.W, A.frame);
+        const float jx = next_rand(rng);
+        const float jy = next_rand(rng);
+        V3 o, d;
+        camera_ray(A.cam.view_inverse, A.cam.projection_inverse, x, y, A.W, A.H, jx, jy, o, d);
+        const float tmin = fmaxf(1.0f, length3(o)) * 1e-3f;
+        const Hit h = trace_ray<false, false>(A.sv, o, d, tmin, 10000.0f, 0xFFu, stack + threadIdx.x, kProgBlock, tc);
+        if (A.hits_out) store_hit(A.hits_out + pix, h);
+        if (h.inst != SOLHIP_MISS) {
+            const Surf sf = fetch_surface(A.tb, h.inst, h.prim, h.u, h.v);
+            const float sg = sgn(dot(neg(d), sf.nrm));
+            e.pixel = pix;
+            e.px = sf.pos.x;
+            e.py = sf.pos.y;
+            e.pz = sf.pos.z;
+            e.nx = sf.nrm.x * sg;
+            e.ny = sf.nrm.y * sg;
+            e.nz = sf.nrm.z * sg;
+            e.rng = rng;
+            hit = true;
+        } else {
+            // nothing to occlude: value 1
+            float4 *px = A.accum + pix;
+            const float4 old = *px;
+            *px = make_float4(mixf(old.x, 1.0f, A.alpha), mixf(old.y, 1.0f, A.alpha), mixf(old.z, 1.0f, A.alpha),
+                              1.0f);
+            if (A.hits_out) {
+                solhip_hit m;
+                m.t = -1.0f;
+                m.u = 0.0f;
+                m.v = 0.0f;
+                m.inst = SOLHIP_MISS;
+                m.prim = SOLHIP_MISS;
+                solhip_hit *dst = A.hits_out + (uint64_t)A.W * A.H + (uint64_t)pix * A.n_ao;
+                for (uint32_t k = 0; k < A.n_ao; ++k) dst[k] = m;
+            }
+        }
+    }
+    // compact the hits into the AO queue: one atomic per wave
+    const uint64_t mask = __ballot(hit);
+    if (mask) {
+        const uint32_t lane = threadIdx.x & 63;
+        uint32_t base = 0;
+        if (lane == __builtin_ctzll(mask)) base = atomicAdd(A.qcount, (uint32_t)__popcll(mask));
+        base = __shfl(base, __builtin_ctzll(mask));
+        if (hit) {
+            const uint32_t slot = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+            A.queue[slot] = e;
+        }
+    }
+    count_rays(A.counters, active ? 1u : 0u, tc.overflow);
+}
+
+// one lane per (queue entry, AO ray); n_ao is a power of two <= 64, so the
+// rays of one pixel are adjacent lanes of one wave
+__global__ void __launch_bounds__(kProgBlock) k_ao_rays(ProgArgs A)
+{
+    __shared__ uint2 stack[kLdsStack * kProgBlock];
+    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
+    const uint64_t t = (uint64_t)lb * kProgBlock + threadIdx.x;
+    const uint32_t count = *A.qcount;
+    const uint32_t shift = __builtin_ctz(A.n_ao);
+    const uint64_t q = t >> shift;
+    const uint32_t k = (uint32_t)(t & (A.n_ao - 1));
+    const bool active = q < count;
+    TravCounts tc{0, 0, 0, false};
+    bool occl = false;
+    uint32_t pix = 0;
+    if (active) {
+        const AoEntry e = A.queue[q];
+        pix = e.pixel;
+        uint32_t rng = pcg_advance(e.rng, 2u * k);
+        const float xi0 = next_rand(rng);
+        const float xi1 = next_rand(rng);
+        const V3 dir = sample_cosine(v3(e.nx, e.ny, e.nz), xi0, xi1);
+        const V3 o = v3(e.px, e.py, e.pz);
+        Hit h;
+        if (A.ao_any)
+            h = trace_ray<true, false>(A.sv, o, dir, A.ao_tmin, A.ao_tmax, 0xFFu, stack + threadIdx.x, kProgBlock, tc);
+        else
+            h = trace_ray<false, false>(A.sv, o, dir, A.ao_tmin, A.ao_tmax, 0xFFu, stack + threadIdx.x, kProgBlock, tc);
+        occl = h.inst != SOLHIP_MISS;
+        if (A.hits_out) store_hit(A.hits_out + (uint64_t)A.W * A.H + (uint64_t)pix * A.n_ao + k, h);
+    }
+    const uint64_t m = __ballot(occl);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t group = lane & ~(A.n_ao - 1);
+    const uint64_t gmask = (A.n_ao == 64) ? ~0ull : (((1ull << A.n_ao) - 1ull) << group);
+    if (active && k == 0) {
+        const uint32_t occ = __popcll(m & gmask);
+        const float value = 1.0f - (float)occ / (float)A.n_ao;
+        float4 *px = A.accum + pix;
+        const float4 old = *px;
+        *px = make_float4(mixf(old.x, value, A.alpha), mixf(old.y, value, A.alpha), mixf(old.z, value, A.alpha), 1.0f);
+    }
+    count_rays(A.counters, active ? 1u : 0u, tc.overflow);
+}
+
+int launch_ao(hipStream_t s, const ProgArgs &a)
+{
+    const uint32_t tiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
+    hipLaunchKernelGGL(k_ao_primary, dim3(tiles), dim3(kProgBlock), 0, s, a);
+    const uint64_t lanes = (uint64_t)a.W * a.H * a.n_ao;
+    hipLaunchKernelGGL(k_ao_rays, dim3((uint32_t)((lanes + kProgBlock - 1) / kProgBlock)), dim3(kProgBlock), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(std::string("AO launch: ") + hipGetErrorString(e));
+        return SOLHIP_ERROR_DEVICE;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// ao.rgen / ao.rchit / ao.rmiss
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kProgBlock) k_ao_glsl(ProgArgs A)
+{
+    __shared__ uint2 stack[kLdsStack * kProgBlock];
+    uint32_t x, y;
+    const bool active = tile_pixel(A.W, A.H, x, y);
+    TravCounts tc{0, 0, 0, false};
+    uint32_t rays = 0;
+    if (active) {
+        const int max_samples = 4, sample_count = 4;  // ao.rgen:42-43
+        float ao = 0.0f;
+        uint32_t rng = tea(x + y * A.W, A.frame);     // ao.rgen:45
+        for (int i = 0; i < sample_count; i++) {
+            const float jx = next_rand(rng);
+            const float jy = next_rand(rng);          // ao.rgen:49
+            V3 o, d;
+            camera_ray(A.cam.view_inverse, A.cam.projection_inverse, x, y, A.W, A.H, jx, jy, o, d);
+            float hitValue = 0.0f;                    // preparePayload, ao.rgen:26-35
+            uint32_t depth = 0, done = 0;
+            const uint32_t sampleId = (uint32_t)i;
+            float rmin = fmaxf(1.0f, length3(o)) * 1e-3f, rmax = 10000.0f;
+            for (;;) {
+                const Hit h = trace_ray<false, false>(A.sv, o, d, rmin, rmax, 0xFFu, stack + threadIdx.x,
+                                                      kProgBlock, tc);
+                ++rays;
+                if (h.inst == SOLHIP_MISS) {
+                    done = 1;                         // ao.rmiss:9
+                } else {
+                    const Surf sf = fetch_surface(A.tb, h.inst, h.prim, h.u, h.v);
+                    const V3 no = v3(sf.pos.x + 0.00001f * d.x, sf.pos.y + 0.00001f * d.y,
+                                     sf.pos.z + 0.00001f * d.z);   // ao.rchit:79
+                    const uint32_t bi = depth + depth * sampleId;   // ao.rchit:80
+                    const float r0 = next_rand(rng), r1 = next_rand(rng);
+                    const float fw = (float)A.blue_w, fh = (float)A.blue_h;
+                    const float fx = (float)x + r0 * fw, fy = (float)y + r1 * fh;
+                    const float mx = fx - fw * floorf(fx / fw), my = fy - fh * floorf(fy / fh);
+                    int cx = (int)mx, cy = (int)my;
+                    if (cx >= (int)A.blue_w) cx -= (int)A.blue_w;
+                    if (cy >= (int)A.blue_h) cy -= (int)A.blue_h;
+                    if (cx < 0) cx += (int)A.blue_w;
+                    if (cy < 0) cy += (int)A.blue_h;
+                    const uint32_t texel = A.blue[(uint64_t)cy * A.blue_w + cx];
+                    const float xi0 = (float)((texel >> (8 * (bi % 4u))) & 0xFFu) / 255.0f;
+                    const float xi1 = (float)((texel >> (8 * ((bi + 1u) % 4u))) & 0xFFu) / 255.0f;
+                    const float sg = sgn(dot(neg(d), sf.nrm));
+                    const V3 hn = scale(sf.nrm, sg);                 // ao.rchit:81
+                    d = sample_cosine(hn, xi0, xi1);                 // ao.rchit:82
+                    o = no;
+                    rmin = 0.001f;
+                    rmax = 10.0f;                                    // ao.rchit:83
+                    if (depth > 0) hitValue += 1.0f;                 // ao.rchit:84-86
+                    depth++;
+                }
+                if (done == 1 || depth > (uint32_t)max_samples) break;   // ao.rgen:71
+            }
+            ao += 1.0f / (float)max_samples * hitValue;             // ao.rgen:74
+        }
+        const float color = 1.0f - ao / (float)sample_count;         // ao.rgen:76
+        float4 *px = A.accum + (uint64_t)y * A.W + x;
+        const float4 old = *px;
+        *px = make_float4(mixf(old.x, color, A.alpha), mixf(old.y, color, A.alpha), mixf(old.z, color, A.alpha), 1.0f);
+    }
+    count_rays(A.counters, rays, tc.overflow);
+}
+
+int launch_ao_glsl(hipStream_t s, const ProgArgs &a)
+{
+    const uint32_t tiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
+    hipLaunchKernelGGL(k_ao_glsl, dim3(tiles), dim3(kProgBlock), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(std::string("AO-glsl launch: ") + hipGetErrorString(e));
+        return SOLHIP_ERROR_DEVICE;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// pathtrace.rgen / pathtrace.rchit / pathtrace.rmiss
+// ---------------------------------------------------------------------------
+// pathtrace.rmiss:8-21
+__device__ __forceinline__ V3 pt_miss(uint32_t enable_sky, V3 d)
+{
+    if (enable_sky) {
+        const V3 wI = normalize(d);
+        const float e = 0.5f * (wI.y + 1.0f);
+        const float tt = saturate((e - 0.35f) / (0.65f - 0.35f));
+        const float t = tt * tt * (3.0f - 2.0f * tt);
+        const V3 sd = normalize(v3(0.0f, 1.0f, -0.25f));
+        const float isSun = dot(wI, sd) > 0.99f ? 1.0f : 0.0f;
+        return v3(mixf(mixf(0.58f, 0.3f, t), 120.0f, isSun), mixf(mixf(0.45f, 0.4f, t), 100.0f, isSun),
+                  mixf(mixf(0.25f, 0.5f, t), 50.0f, isSun));
+    }
+    return v3(0.0f, 0.0f, 0.0f);
+}
+
+__global__ void __launch_bounds__(kProgBlock) k_pathtrace_glsl(ProgArgs A)
+{
+    __shared__ uint2 stack[kLdsStack * kProgBlock];
+    uint32_t x, y;
+    const bool active = tile_pixel(A.W, A.H, x, y);
+    TravCounts tc{0, 0, 0, false};
+    uint32_t rays = 0;
+    if (active) {
+        const int maxBounces = (int)A.max_depth, sampleCount = (int)A.spp;
+        float pc[3] = {0.0f, 0.0f, 0.0f};
+        uint32_t rng = tea(x + y * A.W, A.frame);   // pathtrace.rgen:47
+        for (int i = 0; i < sampleCount; i++) {
+            const float jx = next_rand(rng);
+            const float jy = next_rand(rng);        // :52
+            V3 o, d;
+            camera_ray(A.cam.view_inverse, A.cam.projection_inverse, x, y, A.W, A.H, jx, jy, o, d);
+            const float rmin = fmaxf(1.0f, length3(o)) * 1e-3f, rmax = 10000.0f;   // :35 (never reset)
+            uint32_t depth = 0, done = 0;
+            float acc[3] = {1.0f, 1.0f, 1.0f};
+            for (;;) {
+                const Hit h = trace_ray<false, false>(A.sv, o, d, rmin, rmax, 0xFFu, stack + threadIdx.x,
+                                                      kProgBlock, tc);
+                ++rays;
+                V3 hv;
+                if (h.inst == SOLHIP_MISS) {
+                    hv = pt_miss(A.enable_sky, d);
+                    done = 1;
+                } else {
+                    const float *mat = reinterpret_cast<const float *>(A.tb.material_table[h.inst]);   // :69
+                    const float e0 = mat[4], e1 = mat[5], e2 = mat[6];
+                    if (e0 >= 1.0f || e1 >= 1.0f || e2 >= 1.0f) {       // :71-76
+                        hv = v3(e0, e1, e2);
+                        done = 1;
+                        depth++;
+                    } else {
+                        const Surf sf = fetch_surface(A.tb, h.inst, h.prim, h.u, h.v);
+                        const V3 wI = normalize(d);                      // :90
+                        const float sg = sgn(dot(sf.nrm, neg(wI)));
+                        const V3 nO = scale(sf.nrm, sg);
+                        const float metallic = mat[8], roughness = mat[9];
+                        const float alphaSquared = roughness * roughness;
+                        const float xi0 = next_rand(rng);
+                        const float xi1 = next_rand(rng);                // :93
+                        const float rnd = next_rand(rng);                // :94
+                        o = v3(sf.pos.x + 0.0001f * nO.x, sf.pos.y + 0.0001f * nO.y, sf.pos.z + 0.0001f * nO.z);
+                        const V3 base = v3(mat[0] * sf.col.x, mat[1] * sf.col.y, mat[2] * sf.col.z);
+                        V3 nd;
+                        if (rnd < metallic) {                            // :97-100
+                            nd = sample_ggx(reflect(d, nO), xi0, xi1, alphaSquared);
+                            hv = base;
+                        } else {
+                            const V3 m = sample_ggx(nO, xi0, xi1, alphaSquared);   // :102
+                            if (rnd < fresnel_dielectric(nO, m, 1.0f / 1.5f)) {
+                                nd = reflect(d, m);
+                                hv = v3(1.0f, 1.0f, 1.0f);
+                            } else {
+                                nd = sample_cosine(nO, xi0, xi1);
+                                hv = base;
+                            }
+                        }
+                        d = nd;
+                        depth++;                                         // :112
+                    }
+                }
+                acc[0] = acc[0] * hv.x;                                  // pathtrace.rgen:77
+                acc[1] = acc[1] * hv.y;
+                acc[2] = acc[2] * hv.z;
+                if (done == 1) break;
+                if ((int)depth > maxBounces) {
+                    acc[0] = acc[1] = acc[2] = 0.0f;
+                    break;
+                }
+            }
+            pc[0] = pc[0] + acc[0];
+            pc[1] = pc[1] + acc[1];
+            pc[2] = pc[2] + acc[2];
+        }
+        const float inv = 1.0f / (float)sampleCount;                     // :88
+        pc[0] = inv * pc[0];
+        pc[1] = inv * pc[1];
+        pc[2] = inv * pc[2];
+        float4 *px = A.accum + (uint64_t)y * A.W + x;
+        const float4 old = *px;
+        float m0 = mixf(old.x, pc[0], A.alpha), m1 = mixf(old.y, pc[1], A.alpha), m2 = mixf(old.z, pc[2], A.alpha);
+        const bool bad = isnan(m0) || isnan(m1) || isnan(m2) || isinf(m0) || isinf(m1) || isinf(m2);
+        if (bad) {                                                       // :94-99
+            m0 = old.x;
+            m1 = old.y;
+            m2 = old.z;
+        }
+        *px = make_float4(m0, m1, m2, 1.0f);
+        if (A.out_rgba8) {
+            const float g = 1.0f / 2.2f;                                 // :102
+            const uint32_t r8 = to_unorm8(powf(m0, g)), g8 = to_unorm8(powf(m1, g)), b8 = to_unorm8(powf(m2, g));
+            A.out_rgba8[(uint64_t)y * A.W + x] = r8 | (g8 << 8) | (b8 << 16) | (255u << 24);
+        }
+    }
+    count_rays(A.counters, rays, tc.overflow);
+}
+
+int launch_pathtrace_glsl(hipStream_t s, const ProgArgs &a)
+{
+    const uint32_t tiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
+    hipLaunchKernelGGL(k_pathtrace_glsl, dim3(tiles), dim3(kProgBlock), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(std::string("pathtrace launch: ") + hipGetErrorString(e));
+        return SOLHIP_ERROR_DEVICE;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// debug.rgen / debug.rchit / debug.rmiss
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kProgBlock) k_debug(ProgArgs A)
+{
+    __shared__ uint2 stack[kLdsStack * kProgBlock];
+    uint32_t x, y;
+    const bool active = tile_pixel(A.W, A.H, x, y);
+    TravCounts tc{0, 0, 0, false};
+    if (active) {
+        V3 o, d;
+        camera_ray(A.cam.view_inverse, A.cam.projection_inverse, x, y, A.W, A.H, 0.5f, 0.5f, o, d);  // debug.rgen:20-26
+        const Hit h = trace_ray<false, false>(A.sv, o, d, 0.001f, 1000.0f, 0xFFu, stack + threadIdx.x, kProgBlock, tc);
+        float4 r;
+        if (h.inst == SOLHIP_MISS)
+            r = make_float4(d.x, d.y, d.z, 0.0f);                       // debug.rgen:28
+        else
+            r = make_float4(1.0f - h.u - h.v, h.u, h.v, 0.0f);          // debug.rchit:11-12
+        A.accum[(uint64_t)y * A.W + x] = r;
+    }
+    count_rays(A.counters, active ? 1u : 0u, tc.overflow);
+}
+
+int launch_debug(hipStream_t s, const ProgArgs &a)
+{
+    const uint32_t tiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
+    hipLaunchKernelGGL(k_debug, dim3(tiles), dim3(kProgBlock), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(std::string("debug launch: ") + hipGetErrorString(e));
+        return SOLHIP_ERROR_DEVICE;
+    }
+    return 0;
+}
+
+}  // namespace solhip

@@ -1,0 +1,310 @@
+"""ctypes binding of libsolhip.so (include/solhip.h).
+
+There is no fallback: if the library is missing or no gfx950 device is
+present, every entry point raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .records import HIT, INSTANCE_DESC, MATERIAL, RAY, SCENE_INSTANCE, UNIFORMS
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "..", "lib", "libsolhip.so"))
+HEADER_PATH = os.path.normpath(os.path.join(_HERE, "..", "..", "..", "include", "solhip.h"))
+
+PROG_AO_GLSL, PROG_PATHTRACE_GLSL, PROG_DEBUG, PROG_AO = 1, 2, 3, 4
+TRACE_TERMINATE_ON_FIRST_HIT = 0x4
+TRACE_COUNT = 0x100
+
+ERRORS = {0: "OK", -1: "INVALID_ARGUMENT", -2: "INVALID_HANDLE", -3: "OUT_OF_MEMORY", -4: "NO_DEVICE",
+          -5: "DEVICE", -6: "UNSUPPORTED", -7: "OVERFLOW"}
+
+
+class SolhipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (ERRORS.get(code, "?"), code, msg))
+        self.code = code
+
+
+class Geometry(C.Structure):
+    _fields_ = [("vertex_addr", C.c_uint64), ("vertex_stride", C.c_uint64), ("first_vertex", C.c_uint32),
+                ("vertex_count", C.c_uint32), ("index_addr", C.c_uint64), ("index_byte_offset", C.c_uint64),
+                ("index_count", C.c_uint32), ("opaque", C.c_uint32)]
+
+
+class SceneTables(C.Structure):
+    _fields_ = [("instances_addr", C.c_uint64), ("instance_count", C.c_uint32), ("index_size", C.c_uint32),
+                ("vertex_table_addr", C.c_uint64), ("index_table_addr", C.c_uint64),
+                ("material_table_addr", C.c_uint64), ("object_count", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class Spec(C.Structure):
+    _fields_ = [("enable_sky", C.c_uint32), ("spp_per_frame", C.c_uint32), ("max_depth", C.c_uint32),
+                ("n_ao", C.c_uint32), ("ao_tmin", C.c_float), ("ao_tmax", C.c_float), ("ao_any_hit", C.c_uint32),
+                ("reserved", C.c_uint32 * 9)]
+
+
+class Images(C.Structure):
+    _fields_ = [("accum", C.c_uint64), ("out_rgba8", C.c_uint64), ("blue_noise", C.c_uint64),
+                ("blue_w", C.c_uint32), ("blue_h", C.c_uint32), ("hits_out", C.c_uint64)]
+
+
+class Counters(C.Structure):
+    _fields_ = [("rays", C.c_uint64), ("node_fetches", C.c_uint64), ("tri_fetches", C.c_uint64),
+                ("inst_fetches", C.c_uint64), ("overflow", C.c_uint64)]
+
+
+class BvhInfo(C.Structure):
+    _fields_ = [("nodes", C.c_uint32), ("prims", C.c_uint32), ("depth", C.c_uint32), ("leaf_max", C.c_uint32),
+                ("bytes", C.c_uint64)]
+
+
+# every entry point of include/solhip.h with its ctypes signature
+_u32, _u64, _i32, _vp = C.c_uint32, C.c_uint64, C.c_int32, C.c_void_p
+SIGNATURES = {
+    "solhip_abi_version": (_u32, []),
+    "solhip_last_error": (_i32, [C.c_char_p, C.c_size_t]),
+    "solhip_context_create": (_i32, [_i32, _u32, C.POINTER(_u64)]),
+    "solhip_context_destroy": (_i32, [_u64]),
+    "solhip_stream_create": (_i32, [_u64, C.POINTER(_u64)]),
+    "solhip_stream_destroy": (_i32, [_u64, _u64]),
+    "solhip_stream_sync": (_i32, [_u64, _u64]),
+    "solhip_buffer_create": (_i32, [_u64, _u64, C.POINTER(_u64)]),
+    "solhip_buffer_destroy": (_i32, [_u64, _u64]),
+    "solhip_buffer_upload": (_i32, [_u64, _u64, _u64, _vp, _u64]),
+    "solhip_buffer_download": (_i32, [_u64, _u64, _u64, _vp, _u64]),
+    "solhip_buffer_fill": (_i32, [_u64, _u64, _u64, _u32]),
+    "solhip_buffer_address": (_i32, [_u64, _u64, C.POINTER(_u64)]),
+    "solhip_blas_build": (_i32, [_u64, _u64, C.POINTER(Geometry), _u32, _u32, C.POINTER(_u64)]),
+    "solhip_blas_rebuild": (_i32, [_u64, _u64, _u64, C.POINTER(Geometry), _u32]),
+    "solhip_blas_destroy": (_i32, [_u64, _u64]),
+    "solhip_tlas_build": (_i32, [_u64, _u64, _vp, _u32, C.POINTER(_u64)]),
+    "solhip_tlas_rebuild": (_i32, [_u64, _u64, _u64, _vp, _u32]),
+    "solhip_tlas_destroy": (_i32, [_u64, _u64]),
+    "solhip_blas_info": (_i32, [_u64, _u64, C.POINTER(BvhInfo)]),
+    "solhip_tlas_info": (_i32, [_u64, _u64, C.POINTER(BvhInfo)]),
+    "solhip_trace_rays": (_i32, [_u64, _u64, _u64, _u64, _u64, _u32, _u32]),
+    "solhip_spec_default": (None, [_u32, C.POINTER(Spec)]),
+    "solhip_program_create": (_i32, [_u64, _u32, C.POINTER(Spec), C.POINTER(_u64)]),
+    "solhip_program_destroy": (_i32, [_u64, _u64]),
+    "solhip_trace": (_i32, [_u64, _u64, _u64, _u64, C.POINTER(SceneTables), _vp, _i32, C.POINTER(Images), _u32, _u32]),
+    "solhip_stats": (_i32, [_u64, C.POINTER(Counters), _u32]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libsolhip.so not built (%s): run __graft_entry__.build()" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error():
+    buf = C.create_string_buffer(1024)
+    lib().solhip_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(rc):
+    if rc != 0:
+        raise SolhipError(rc, last_error())
+    return rc
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def spec_default(kind):
+    s = Spec()
+    lib().solhip_spec_default(kind, C.byref(s))
+    return s
+
+
+class Buffer:
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        h = C.c_uint64()
+        check(lib().solhip_buffer_create(ctx.h, self.nbytes, C.byref(h)))
+        self.h = h.value
+        a = C.c_uint64()
+        check(lib().solhip_buffer_address(ctx.h, self.h, C.byref(a)))
+        self.addr = a.value
+
+    def upload(self, arr, offset=0):
+        arr = np.ascontiguousarray(arr)
+        check(lib().solhip_buffer_upload(self.ctx.h, self.h, offset, _ptr(arr), arr.nbytes))
+
+    def download(self, dtype, count=None, offset=0):
+        dtype = np.dtype(dtype)
+        if count is None:
+            count = (self.nbytes - offset) // dtype.itemsize
+        out = np.empty(count, dtype)
+        check(lib().solhip_buffer_download(self.ctx.h, self.h, offset, _ptr(out), out.nbytes))
+        return out
+
+    def fill(self, value=0, stream=0):
+        check(lib().solhip_buffer_fill(self.ctx.h, stream, self.h, value))
+
+    def destroy(self):
+        if self.h:
+            check(lib().solhip_buffer_destroy(self.ctx.h, self.h))
+            self.h = 0
+
+
+class Context:
+    def __init__(self, device=0):
+        h = C.c_uint64()
+        check(lib().solhip_context_create(device, 0, C.byref(h)))
+        self.h = h.value
+
+    def close(self):
+        if self.h:
+            check(lib().solhip_context_destroy(self.h))
+            self.h = 0
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def buffer(self, nbytes):
+        return Buffer(self, nbytes)
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        b = Buffer(self, max(arr.nbytes, 16))
+        if arr.nbytes:
+            b.upload(arr)
+        return b
+
+    def sync(self, stream=0):
+        check(lib().solhip_stream_sync(self.h, stream))
+
+    def stream(self):
+        h = C.c_uint64()
+        check(lib().solhip_stream_create(self.h, C.byref(h)))
+        return h.value
+
+    def blas_build(self, vbuf, stride, vertex_count, ibuf, index_count, first_vertex=0, index_byte_offset=0,
+                   stream=0):
+        g = Geometry(vbuf.addr, stride, first_vertex, vertex_count, ibuf.addr, index_byte_offset, index_count, 1)
+        h = C.c_uint64()
+        check(lib().solhip_blas_build(self.h, stream, C.byref(g), 1, 0, C.byref(h)))
+        return h.value
+
+    def blas_rebuild(self, blas, vbuf, stride, vertex_count, ibuf, index_count, stream=0):
+        g = Geometry(vbuf.addr, stride, 0, vertex_count, ibuf.addr, 0, index_count, 1)
+        check(lib().solhip_blas_rebuild(self.h, stream, blas, C.byref(g), 1))
+
+    def tlas_build(self, instances, stream=0):
+        inst = np.ascontiguousarray(instances, INSTANCE_DESC)
+        h = C.c_uint64()
+        check(lib().solhip_tlas_build(self.h, stream, _ptr(inst) if len(inst) else None, len(inst), C.byref(h)))
+        return h.value
+
+    def tlas_rebuild(self, tlas, instances, stream=0):
+        inst = np.ascontiguousarray(instances, INSTANCE_DESC)
+        check(lib().solhip_tlas_rebuild(self.h, stream, tlas, _ptr(inst) if len(inst) else None, len(inst)))
+
+    def blas_info(self, blas):
+        i = BvhInfo()
+        check(lib().solhip_blas_info(self.h, blas, C.byref(i)))
+        return i
+
+    def tlas_info(self, tlas):
+        i = BvhInfo()
+        check(lib().solhip_tlas_info(self.h, tlas, C.byref(i)))
+        return i
+
+    def trace_rays_dev(self, tlas, ray_buf, hit_buf, n, flags=0, stream=0):
+        check(lib().solhip_trace_rays(self.h, stream, tlas, ray_buf.h, hit_buf.h, n, flags))
+
+    def trace_rays(self, tlas, rays, flags=0):
+        rays = np.ascontiguousarray(rays, RAY)
+        rb = self.upload(rays)
+        hb = self.buffer(max(16, len(rays) * HIT.itemsize))
+        try:
+            self.trace_rays_dev(tlas, rb, hb, len(rays), flags)
+            self.sync()
+            return hb.download(HIT, len(rays))
+        finally:
+            rb.destroy()
+            hb.destroy()
+
+    def program(self, kind, spec=None):
+        h = C.c_uint64()
+        check(lib().solhip_program_create(self.h, kind, C.byref(spec) if spec is not None else None, C.byref(h)))
+        return h.value
+
+    def program_destroy(self, p):
+        check(lib().solhip_program_destroy(self.h, p))
+
+    def trace(self, prog, tlas, tables, uniforms, accum_start, images, W, H, stream=0):
+        u = np.ascontiguousarray(uniforms, UNIFORMS)
+        check(lib().solhip_trace(self.h, stream, prog, tlas, C.byref(tables) if tables is not None else None,
+                                 _ptr(u), accum_start, C.byref(images), W, H))
+
+    def stats(self, reset=False):
+        c = Counters()
+        check(lib().solhip_stats(self.h, C.byref(c), 1 if reset else 0))
+        return c
+
+
+class GpuScene:
+    """Uploads a SceneData (sol-rs_amd/python/solhip/scenes.py) and builds its
+    BLASes and TLAS: what SceneDescription::from_scene does for the Vulkan
+    path (src/ray/mod.rs:59-156)."""
+
+    def __init__(self, ctx, sd, index_size=4):
+        self.ctx = ctx
+        self.sd = sd
+        self.bufs = []
+        self.blas = []
+        for b in sd.blas:
+            vb = ctx.upload(b.vertices)
+            ib = ctx.upload(b.indices.astype(np.uint32))
+            self.bufs += [vb, ib]
+            self.blas.append(ctx.blas_build(vb, 64, len(b.vertices), ib, len(b.indices)))
+        descs = sd.instance_descs.copy()
+        descs["blas"] = [self.blas[int(k)] for k in sd.instance_descs["blas"]] if len(descs) else []
+        self.descs = descs
+        self.tlas = ctx.tlas_build(descs)
+        # hit-shader tables
+        self.inst_buf = ctx.upload(np.ascontiguousarray(sd.scene_instances, SCENE_INSTANCE))
+        vaddr, iaddr = [], []
+        itype = np.uint64 if index_size == 8 else np.uint32
+        for verts, inds in sd.objects:
+            vb = ctx.upload(verts)
+            ib = ctx.upload(np.asarray(inds).astype(itype))
+            self.bufs += [vb, ib]
+            vaddr.append(vb.addr)
+            iaddr.append(ib.addr)
+        self.vt = ctx.upload(np.array(vaddr or [0], np.uint64))
+        self.it = ctx.upload(np.array(iaddr or [0], np.uint64))
+        mats = np.ascontiguousarray(sd.materials, MATERIAL)
+        self.mat_buf = ctx.upload(mats if len(mats) else np.zeros(1, MATERIAL))
+        maddr = [self.mat_buf.addr + int(m) * MATERIAL.itemsize for m in sd.instance_material]
+        self.mt = ctx.upload(np.array(maddr or [0], np.uint64))
+        self.bufs += [self.inst_buf, self.vt, self.it, self.mat_buf, self.mt]
+        self.tables = SceneTables(self.inst_buf.addr, len(sd.scene_instances), index_size, self.vt.addr, self.it.addr,
+                                  self.mt.addr, len(sd.objects), 0)
+
+    def update_instances(self, instance_descs):
+        descs = instance_descs.copy()
+        descs["blas"] = [self.blas[int(k)] for k in instance_descs["blas"]]
+        self.descs = descs
+        self.ctx.tlas_rebuild(self.tlas, descs)

@@ -1,0 +1,234 @@
+// bvh_cpu.hip -- host replay of the library's BVH build and traversal steps
+// (bvh_logic.h, traverse.h) for CPU tests.  Compiled for the host only by
+// tests/cpu_harness/build.py; it exercises the same per-item code the GPU
+// kernels run, with a sequential sort / refit / allocator in place of the
+// device-wide ones.  Test infrastructure: not part of libsolhip.
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "../../sol-rs_amd/csrc/bvh_logic.h"
+#include "../../sol-rs_amd/csrc/traverse.h"
+
+using namespace solhip;
+
+namespace {
+
+struct HostBvh {
+    std::vector<WideNode> nodes;
+    std::vector<uint32_t> leaf_ids;
+    uint32_t depth = 0;
+    float root_box[6];
+};
+
+// prim boxes: n x (lo3, hi3)
+int build(const float *pbox, uint32_t n, uint32_t leaf_max, HostBvh &out)
+{
+    out.nodes.clear();
+    out.leaf_ids.assign(n, 0);
+    if (n == 0) return 0;
+    float blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            const float c = 0.5f * pbox[6 * i + a] + 0.5f * pbox[6 * i + 3 + a];
+            blo[a] = fminf(blo[a], c);
+            bhi[a] = fmaxf(bhi[a], c);
+        }
+    std::vector<uint32_t> keys(n), vals(n);
+    for (uint32_t i = 0; i < n; ++i) keys[i] = morton30(pbox + 6 * i, pbox + 6 * i + 3, blo, bhi);
+    std::iota(vals.begin(), vals.end(), 0u);
+    std::stable_sort(vals.begin(), vals.end(), [&](uint32_t a, uint32_t b) { return keys[a] < keys[b]; });
+    std::vector<uint32_t> skeys(n);
+    for (uint32_t i = 0; i < n; ++i) skeys[i] = keys[vals[i]];
+    const uint32_t nb = 2 * n - 1;
+    std::vector<uint32_t> left(n), right(n), parent(nb, 0xFFFFFFFFu), rfirst(n), ncount(nb, 0);
+    std::vector<float4> nbox(2 * (size_t)nb);
+    for (int i = 0; i + 1 < (int)n; ++i) karras_node(skeys.data(), (int)n, i, left.data(), right.data(), parent.data(), rfirst.data());
+    if (n > 1) parent[0] = 0xFFFFFFFFu;
+    // every node except the root must have exactly one parent
+    for (uint32_t k = 1; k < nb; ++k)
+        if (n > 1 && parent[k] >= n - 1) return -10;
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t p = vals[k];
+        const uint32_t node = n - 1 + k;
+        nbox[2 * node] = make_float4(pbox[6 * p], pbox[6 * p + 1], pbox[6 * p + 2], 0.0f);
+        nbox[2 * node + 1] = make_float4(pbox[6 * p + 3], pbox[6 * p + 4], pbox[6 * p + 5], 0.0f);
+        ncount[node] = 1;
+    }
+    // internal nodes: children have larger indices than... not guaranteed, so
+    // resolve by repeated passes over a post-order from the root
+    if (n > 1) {
+        std::vector<uint32_t> order, stack{0u};
+        while (!stack.empty()) {
+            const uint32_t x = stack.back();
+            stack.pop_back();
+            order.push_back(x);
+            if (x < n - 1) {
+                stack.push_back(left[x]);
+                stack.push_back(right[x]);
+            }
+        }
+        if (order.size() != nb) return -11;
+        for (auto it = order.rbegin(); it != order.rend(); ++it) {
+            const uint32_t x = *it;
+            if (x >= n - 1) continue;
+            const float4 al = nbox[2 * left[x]], ah = nbox[2 * left[x] + 1];
+            const float4 bl = nbox[2 * right[x]], bh = nbox[2 * right[x] + 1];
+            nbox[2 * x] = make_float4(fminf(al.x, bl.x), fminf(al.y, bl.y), fminf(al.z, bl.z), 0.0f);
+            nbox[2 * x + 1] = make_float4(fmaxf(ah.x, bh.x), fmaxf(ah.y, bh.y), fmaxf(ah.z, bh.z), 0.0f);
+            ncount[x] = ncount[left[x]] + ncount[right[x]];
+        }
+    }
+    const float4 rl = nbox[0], rh = nbox[1];
+    const float rb[6] = {rl.x, rl.y, rl.z, rh.x, rh.y, rh.z};
+    std::memcpy(out.root_box, rb, sizeof(rb));
+    out.nodes.assign(n, WideNode{});
+    std::vector<uint32_t> wsrc(n, 0);
+    uint32_t counters[3] = {1u, 0u, 0u};
+    CollapseArgs A;
+    A.n = (int)n;
+    A.leaf_max = leaf_max;
+    A.node_cap = n;
+    A.left = left.data();
+    A.right = right.data();
+    A.rfirst = rfirst.data();
+    A.ncount = ncount.data();
+    A.nbox = nbox.data();
+    A.vals = vals.data();
+    A.wnodes = out.nodes.data();
+    A.wsrc = wsrc.data();
+    A.counters = counters;
+    A.leaf_ids = out.leaf_ids.data();
+    uint32_t begin = 0, end = 1;
+    out.depth = 0;
+    auto alloc = [&](int which, uint32_t amount) {
+        const uint32_t old = counters[which];
+        counters[which] += amount;
+        return old;
+    };
+    while (begin < end) {
+        A.begin = begin;
+        A.end = end;
+        for (uint32_t w = begin; w < end; ++w)
+            if (collapse_node(A, w, alloc)) return -12;
+        begin = end;
+        end = counters[0];
+        ++out.depth;
+        if (end > n) return -13;
+    }
+    if (counters[1] != n) return -14;
+    out.nodes.resize(end);
+    for (uint32_t w = 0; w < end; ++w)
+        if (!node_valid(out.nodes[w], w, end, n)) return -15;
+    return 0;
+}
+
+struct HostBlas {
+    HostBvh bvh;
+    std::vector<TriRec> tris;
+};
+struct HostScene {
+    std::vector<HostBlas> blas;
+    HostBvh tlas;
+    std::vector<InstRec> insts;
+    std::vector<uint32_t> meta;
+};
+
+}  // namespace
+
+extern "C" {
+
+// tris: per BLAS, 9 floats per triangle; inst: 12-float row-major transforms,
+// blas index, mask.  Returns a handle (heap pointer) or null on error.
+void *hc_scene_create(const float *const *tris, const uint32_t *ntris, uint32_t nblas, const float *xf,
+                      const uint32_t *inst_blas, const uint32_t *inst_mask, uint32_t ninst, int *err)
+{
+    HostScene *s = new HostScene();
+    s->blas.resize(nblas);
+    for (uint32_t b = 0; b < nblas; ++b) {
+        const uint32_t n = ntris[b];
+        std::vector<float> pbox(6 * (size_t)n);
+        for (uint32_t i = 0; i < n; ++i)
+            for (int a = 0; a < 3; ++a) {
+                const float *t = tris[b] + 9 * (size_t)i;
+                pbox[6 * i + a] = fminf(fminf(t[a], t[3 + a]), t[6 + a]);
+                pbox[6 * i + 3 + a] = fmaxf(fmaxf(t[a], t[3 + a]), t[6 + a]);
+            }
+        int rc = build(pbox.data(), n, kLeafMaxTri, s->blas[b].bvh);
+        if (rc) {
+            *err = rc;
+            delete s;
+            return nullptr;
+        }
+        s->blas[b].tris.resize(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t p = s->blas[b].bvh.leaf_ids[i];
+            TriRec r;
+            std::memcpy(r.v, tris[b] + 9 * (size_t)p, 36);
+            r.prim = p;
+            r.pad[0] = r.pad[1] = 0;
+            s->blas[b].tris[i] = r;
+        }
+    }
+    std::vector<float> pbox(6 * (size_t)ninst);
+    for (uint32_t i = 0; i < ninst; ++i) {
+        const HostBlas &hb = s->blas[inst_blas[i]];
+        instance_box(xf + 12 * (size_t)i, !hb.tris.empty(), hb.bvh.root_box, &pbox[6 * i], &pbox[6 * i + 3]);
+    }
+    int rc = build(pbox.data(), ninst, kLeafMaxInst, s->tlas);
+    if (rc) {
+        *err = rc - 100;
+        delete s;
+        return nullptr;
+    }
+    s->insts.resize(ninst);
+    s->meta.resize(ninst);
+    for (uint32_t i = 0; i < ninst; ++i) {
+        const uint32_t p = s->tlas.leaf_ids[i];
+        InstRec r;
+        inverse3x4(xf + 12 * (size_t)p, r.inv);
+        const HostBlas &hb = s->blas[inst_blas[p]];
+        r.nodes = hb.tris.empty() ? nullptr : hb.bvh.nodes.data();
+        r.tris = hb.tris.data();
+        s->insts[i] = r;
+        s->meta[i] = (p & 0xFFFFFFu) | (inst_mask[p] << 24);
+    }
+    *err = 0;
+    return s;
+}
+
+void hc_scene_destroy(void *h) { delete static_cast<HostScene *>(h); }
+
+void hc_scene_info(void *h, uint32_t blas, uint32_t *nodes, uint32_t *depth)
+{
+    HostScene *s = static_cast<HostScene *>(h);
+    const HostBvh &b = blas == 0xFFFFFFFFu ? s->tlas : s->blas[blas].bvh;
+    *nodes = (uint32_t)b.nodes.size();
+    *depth = b.depth;
+}
+
+// rays: 8 floats each; hits: 5 words each; counts: nodes, tris, insts, overflow
+void hc_trace(void *h, const float *rays, uint32_t n, uint32_t any, void *hits, uint64_t *counts)
+{
+    HostScene *s = static_cast<HostScene *>(h);
+    SceneView sv;
+    sv.tnodes = s->insts.empty() ? nullptr : s->tlas.nodes.data();
+    sv.insts = s->insts.data();
+    sv.inst_meta = s->meta.data();
+    uint2 lds[kLdsStack];
+    for (uint32_t i = 0; i < n; ++i) {
+        const float *r = rays + 8 * (size_t)i;
+        TravCounts tc{0, 0, 0, false};
+        Hit hit = any ? trace_ray<true, true>(sv, v3(r[0], r[1], r[2]), v3(r[4], r[5], r[6]), r[3], r[7], 0xFFu, lds, 1, tc)
+                      : trace_ray<false, true>(sv, v3(r[0], r[1], r[2]), v3(r[4], r[5], r[6]), r[3], r[7], 0xFFu, lds, 1, tc);
+        std::memcpy((char *)hits + 20 * (size_t)i, &hit, 20);
+        counts[0] += tc.nodes;
+        counts[1] += tc.tris;
+        counts[2] += tc.insts;
+        counts[3] += tc.overflow ? 1 : 0;
+    }
+}
+
+}  // extern "C"

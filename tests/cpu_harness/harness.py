@@ -1,0 +1,69 @@
+"""ctypes wrapper of libbvhcpu.so (host replay of the library's BVH build and
+traversal)."""
+import ctypes as C
+import os
+
+import numpy as np
+
+from solhip.records import HIT, RAY
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libbvhcpu.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            from . import build as _b  # noqa
+        L = C.CDLL(LIB)
+        vp = C.c_void_p
+        L.hc_scene_create.restype = vp
+        L.hc_scene_create.argtypes = [vp, vp, C.c_uint32, vp, vp, vp, C.c_uint32, C.POINTER(C.c_int)]
+        L.hc_scene_destroy.argtypes = [vp]
+        L.hc_scene_info.argtypes = [vp, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.hc_trace.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, vp]
+        _lib = L
+    return _lib
+
+
+class HostScene:
+    def __init__(self, sd):
+        self._keep = []
+        nb = len(sd.blas)
+        ptrs = (C.c_void_p * max(1, nb))()
+        counts = np.zeros(max(1, nb), np.uint32)
+        for i, b in enumerate(sd.blas):
+            t = np.ascontiguousarray(b.vertices["pos"][:, :3][b.indices.reshape(-1, 3)].reshape(-1), np.float32)
+            self._keep.append(t)
+            ptrs[i] = t.ctypes.data
+            counts[i] = b.tri_count
+        n = len(sd.instance_descs)
+        xf = np.ascontiguousarray(sd.instance_descs["transform"].reshape(-1), np.float32) if n else np.zeros(12, np.float32)
+        ib = np.ascontiguousarray(sd.instance_descs["blas"].astype(np.uint32)) if n else np.zeros(1, np.uint32)
+        im = np.ascontiguousarray((sd.instance_descs["id_mask"] >> 24).astype(np.uint32)) if n else np.zeros(1, np.uint32)
+        self._keep += [xf, ib, im, counts]
+        err = C.c_int(0)
+        self.h = lib().hc_scene_create(C.cast(ptrs, C.c_void_p), counts.ctypes.data, nb, xf.ctypes.data,
+                                       ib.ctypes.data, im.ctypes.data, n, C.byref(err))
+        if not self.h:
+            raise RuntimeError("host BVH build failed: %d" % err.value)
+
+    def info(self, blas):
+        a, b = C.c_uint32(), C.c_uint32()
+        lib().hc_scene_info(self.h, blas, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def trace(self, rays, any_hit=False):
+        rays = np.ascontiguousarray(rays, RAY)
+        hits = np.zeros(len(rays), HIT)
+        counts = np.zeros(4, np.uint64)
+        lib().hc_trace(self.h, rays.ctypes.data, len(rays), 1 if any_hit else 0, hits.ctypes.data,
+                       counts.ctypes.data)
+        return hits, counts
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().hc_scene_destroy(self.h)
+            self.h = None

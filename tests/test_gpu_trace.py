@@ -1,0 +1,109 @@
+"""GPU parity of the BVH build + traversal (solhip_trace_rays) against the
+oracle's brute-force closest hit.  Bar: hit records (t, u, v, instance,
+primitive) bit-identical, including misses and the (t, inst, prim) tie rule."""
+import numpy as np
+import pytest
+
+import raygen
+import solhip
+from solhip import scenes
+from solhip.records import HIT
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(ctx, oracle, sd, rays, brute=True, flags=0):
+    gs = solhip.GpuScene(ctx, sd)
+    ctx.stats(reset=True)
+    g = ctx.trace_rays(gs.tlas, rays, flags)
+    st = ctx.stats()
+    assert st.overflow == 0
+    sc = oracle.Scene(sd)
+    r = sc.trace(rays, use_bvh=not brute)
+    if flags & solhip.TRACE_TERMINATE_ON_FIRST_HIT:
+        assert np.array_equal(g["inst"] == 0xFFFFFFFF, r["inst"] == 0xFFFFFFFF)
+        return g, r
+    diff = np.nonzero((g.view(np.uint32).reshape(-1, 5) != r.view(np.uint32).reshape(-1, 5)).any(axis=1))[0]
+    assert len(diff) == 0, "%d/%d hits differ, first %s: gpu %s ref %s" % (
+        len(diff), len(rays), diff[:5], g[diff[:3]], r[diff[:3]])
+    return g, r
+
+
+SMALL_SCENES = {
+    "triangle": scenes.single_triangle,
+    "cornell": scenes.cornell_box,
+    "soup": lambda: scenes.random_soup(3000, seed=5),
+    "instances": scenes.random_instances,
+    "heightfield64": lambda: scenes.heightfield(64),
+}
+
+
+@pytest.mark.parametrize("name", sorted(SMALL_SCENES))
+def test_trace_random_rays_match_brute_force(ctx, oracle, name):
+    sd = SMALL_SCENES[name]()
+    rays = raygen.random_rays(4000, (0, 0, 0), 2.0 if name != "instances" else 6.0, seed=11)
+    _check(ctx, oracle, sd, rays)
+
+
+@pytest.mark.parametrize("name", sorted(SMALL_SCENES))
+def test_trace_aimed_rays_match_brute_force(ctx, oracle, name):
+    sd = SMALL_SCENES[name]()
+    rays = raygen.aimed_rays(sd, 3000, seed=12)
+    g, r = _check(ctx, oracle, sd, rays)
+    # rays aimed at vertices / shared edges of closed meshes must not leak
+    if name in ("heightfield64",):
+        assert (g["inst"] != 0xFFFFFFFF).mean() > 0.5
+
+
+@pytest.mark.parametrize("name", ["cornell", "heightfield64", "instances"])
+def test_trace_camera_rays_match_brute_force(ctx, oracle, name):
+    sd = SMALL_SCENES[name]()
+    rays = raygen.camera_rays(sd, 48, 32, oracle)
+    _check(ctx, oracle, sd, rays)
+
+
+def test_trace_axis_parallel_rays(ctx, oracle):
+    sd = scenes.cornell_box()
+    _check(ctx, oracle, sd, raygen.axis_rays(3000))
+
+
+def test_trace_terminate_on_first_hit_agrees_on_occlusion(ctx, oracle):
+    sd = scenes.random_soup(2000, seed=9)
+    rays = raygen.random_rays(3000, (0, 0, 0), 1.5, seed=13, tmax=0.5)
+    _check(ctx, oracle, sd, rays, flags=solhip.TRACE_TERMINATE_ON_FIRST_HIT)
+
+
+def test_trace_tmin_tmax_window(ctx, oracle):
+    sd = scenes.random_soup(1500, seed=4)
+    rays = raygen.random_rays(3000, (0, 0, 0), 1.0, seed=14, tmin=0.3, tmax=0.9)
+    _check(ctx, oracle, sd, rays)
+
+
+def test_empty_tlas_misses(ctx):
+    sd = scenes.empty_scene()
+    gs = solhip.GpuScene(ctx, sd)
+    rays = raygen.random_rays(100, (0, 0, 0), 1.0)
+    g = ctx.trace_rays(gs.tlas, rays)
+    assert (g["inst"] == 0xFFFFFFFF).all() and (g["t"] == -1.0).all()
+
+
+def test_bvh_info_and_counts(ctx, oracle):
+    sd = scenes.heightfield(128)
+    gs = solhip.GpuScene(ctx, sd)
+    info = ctx.blas_info(gs.blas[0])
+    assert info.prims == sd.tri_count
+    assert 1 <= info.nodes < info.prims
+    assert info.depth >= 2
+    rays = raygen.camera_rays(sd, 64, 36, oracle)
+    ctx.stats(reset=True)
+    ctx.trace_rays(gs.tlas, rays, solhip.TRACE_COUNT)
+    st = ctx.stats(reset=True)
+    assert st.rays == len(rays) and st.overflow == 0
+    assert st.node_fetches >= len(rays) and st.inst_fetches >= 1
+
+
+def test_heightfield_large_sampled(ctx, oracle):
+    """512^2 heightfield (524k triangles) against the oracle's own BVH"""
+    sd = scenes.heightfield(512)
+    rays = np.concatenate([raygen.camera_rays(sd, 64, 36, oracle), raygen.aimed_rays(sd, 4000, seed=21, dist=0.5)])
+    _check(ctx, oracle, sd, rays, brute=False)
