@@ -1,0 +1,262 @@
+"""Benchmark: ray-traced AO on the 2.1M-triangle heightfield at 1080p, 1 GPU.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--grid 1024] [--res 1920x1080] [--no-cpu]
+
+A step is one frame of the plain AO program (SOLHIP_PROG_AO): one jittered
+primary ray per pixel, 8 cosine AO rays per primary hit (closest hit, range
+[1e-4, 0.1] x extent), running-mean accumulation.  Inputs (BVH, tables,
+accumulation buffer) are resident in HBM before the timed region.  Prints
+one JSON line.
+
+roofline: the dominant kernel (the AO-ray kernel) against the 8 TB/s HBM
+peak, with algorithmic bytes per ray
+    B_ray = 32 (ray) + 20 (hit) + 80 N_node + 36 N_tri + 64 N_inst
+where the fetch counts come from the counting variant of the same kernels on
+the same frame, and the launch duration from hipEvents recorded by the
+library on the launch stream during the timed region.
+
+cpu_baseline: the oracle (oracle/, a C restatement of the reference GLSL with
+its own binary BVH, OpenMP over rows) on whole frames of the same workload,
+as many as fit ~10 s; the GPU accumulation of the same frames is compared
+with it bit for bit.
+
+    python bench.py --copy-calibration BYTES
+
+runs one plain device-to-device copy of BYTES bytes (torch), the reference
+kernel whose FETCH_SIZE / WRITE_SIZE readings scale the counters of the AO
+kernel to bytes.
+"""
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "sol-rs_amd", "python")]
+
+HBM_PEAK_GBS = 8000.0
+B_RAY_FIXED = 32 + 20
+B_NODE, B_TRI, B_INST = 80, 36, 64
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_ao_rays_heightfield1024_1080p.txt")
+
+
+def parse_args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--grid", type=int, default=1024)
+    ap.add_argument("--res", default="1920x1080")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--copy-calibration", type=int, default=0)
+    return ap.parse_args()
+
+
+def copy_calibration(nbytes):
+    import torch
+    n = nbytes // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4}))
+
+
+def bytes_per_ray(c):
+    if c.rays == 0:
+        return 0.0
+    return B_RAY_FIXED + (B_NODE * c.node_fetches + B_TRI * c.tri_fetches + B_INST * c.inst_fetches) / c.rays
+
+
+def read_traffic():
+    """HBM bytes per launch of the AO-ray kernel from the committed rocprofv3
+    --pmc summary (line 'traffic_bytes_per_launch: <int>'), else None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read())
+            return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def main():
+    a = parse_args()
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
+    if a.gpus != 1:
+        print("bench.py measures one GPU only (this build has no multi-GPU path); got --gpus %d" % a.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    W, H = (int(v) for v in a.res.lower().split("x"))
+    import solhip
+    from solhip import scenes
+
+    sd = scenes.heightfield(a.grid)
+    ctx = solhip.Context(0)
+    t0 = time.perf_counter()
+    gs = solhip.GpuScene(ctx, sd)
+    ctx.sync()
+    upload_build_s = time.perf_counter() - t0
+    # BVH build alone (rebuild in place on the resident vertex buffers)
+    vb, ib = gs.bufs[0], gs.bufs[1]
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.blas_rebuild(gs.blas[0], vb, 64, len(sd.blas[0].vertices), ib, len(sd.blas[0].indices))
+    ctx.sync()
+    blas_build_ms = (time.perf_counter() - t0) * 1e3
+    info = ctx.blas_info(gs.blas[0])
+
+    spec = solhip.spec_default(solhip.PROG_AO)
+    spec.n_ao = 8
+    spec.ao_tmin = 1e-4 * sd.extent
+    spec.ao_tmax = 0.1 * sd.extent
+    spec.ao_any_hit = 0
+    prog = ctx.program(solhip.PROG_AO, spec)
+    acc = ctx.upload(np.zeros((H, W, 4), np.float32))
+    img = solhip.Images(acc.h, 0, 0, 0, 0, 0)
+
+    # counting frame (fetch counts per ray, same frame as the timed ones)
+    ctx.profile(solhip.api.PROFILE_COUNT)
+    for k in ("ao_primary", "ao_rays"):
+        ctx.kernel_stats(k, reset=True)
+    ctx.trace(prog, gs.tlas, gs.tables, sd.uniforms(W, H, 0), 0, img, W, H)
+    cp = ctx.kernel_stats("ao_primary", reset=True).counters
+    ca = ctx.kernel_stats("ao_rays", reset=True).counters
+    ctx.profile(0)
+
+    # warmup, then the timed region
+    for i in range(a.warmup):
+        ctx.trace(prog, gs.tlas, gs.tables, sd.uniforms(W, H, i), 0, img, W, H)
+    ctx.sync()
+    ctx.stats(reset=True)
+    for k in ("ao_primary", "ao_rays"):
+        ctx.kernel_stats(k, reset=True)
+    ctx.profile(solhip.api.PROFILE_TIME)
+    unis = [sd.uniforms(W, H, a.warmup + i) for i in range(a.steps)]
+    ctx.sync()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        ctx.trace(prog, gs.tlas, gs.tables, unis[i], a.warmup, img, W, H)
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    ctx.profile(0)
+    kp = ctx.kernel_stats("ao_primary")
+    ka = ctx.kernel_stats("ao_rays")
+    st = ctx.stats(reset=True)
+    assert st.overflow == 0, "traversal stack overflow"
+    rays = st.rays
+    mrays = rays / elapsed / 1e6
+
+    # roofline of the dominant kernel
+    dom_name, dom, dom_c = ("ao_rays", ka, ca) if ka.total_ms >= kp.total_ms else ("ao_primary", kp, cp)
+    avg_launch_s = dom.total_ms / max(1, dom.launches) / 1e3
+    rays_per_launch = dom.counters.rays / max(1, dom.launches)
+    b_ray = bytes_per_ray(dom_c)
+    achieved = b_ray * rays_per_launch / avg_launch_s / 1e9
+    traffic = read_traffic()
+
+    result = {
+        "metric": "Mrays/s on 1 GPU",
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(elapsed / a.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "plain ray-traced AO, %dx%d-quad fBm heightfield (%d triangles, one BLAS), %dx%d, "
+                        "1 primary + %d cosine AO rays per primary hit, closest hit, AO range [%.0e, %.2f]"
+                        % (a.grid, a.grid, sd.tri_count, W, H, spec.n_ao, spec.ao_tmin, spec.ao_tmax),
+            "triangles": sd.tri_count,
+            "resolution": [W, H],
+            "n_ao": spec.n_ao,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom_name,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "bytes_per_ray": round(b_ray, 1),
+            "fetches_per_ray": {
+                "node": round(dom_c.node_fetches / max(1, dom_c.rays), 3),
+                "tri": round(dom_c.tri_fetches / max(1, dom_c.rays), 3),
+                "inst": round(dom_c.inst_fetches / max(1, dom_c.rays), 3),
+            },
+            "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+            "rays_per_launch": int(rays_per_launch),
+        },
+        "extra": {
+            "rays_per_frame": int(rays / a.steps),
+            "primary_hit_fraction": round((ca.rays / spec.n_ao) / (W * H), 4),
+            "kernel_ms_per_frame": {"ao_primary": round(kp.total_ms / max(1, kp.launches), 4),
+                                    "ao_rays": round(ka.total_ms / max(1, ka.launches), 4)},
+            "blas_build_ms": round(blas_build_ms, 2),
+            "scene_upload_and_build_s": round(upload_build_s, 3),
+            "bvh": {"nodes": info.nodes, "depth": info.depth, "bytes": info.bytes},
+            "primary_bytes_per_ray": round(bytes_per_ray(cp), 1),
+        },
+    }
+
+    if not a.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(ctx, gs, sd, spec, W, H, a.cpu_seconds)
+    ctx.close()
+    print(json.dumps(result))
+
+
+def cpu_baseline(ctx, gs, sd, spec, W, H, target_s):
+    """the oracle on whole frames 0..F-1 of the same workload (F sized to about
+    target_s of CPU work, at most 30 frames); the GPU renders the same frames
+    into a fresh accumulation buffer and the two are compared bit for bit"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import solref
+    import solhip
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    sc = solref.Scene(sd)
+    params = solref.AOParams(spec.n_ao, spec.ao_tmin, spec.ao_tmax, 0)
+    acc = np.zeros((H, W, 4), np.float32)
+    t0 = time.perf_counter()
+    nrays, _, _ = sc.render_ao(sd.uniforms(W, H, 0), params, 0, acc, W, H, nthreads=threads)
+    t_first = time.perf_counter() - t0
+    frames = int(max(1, min(30, round(target_s / max(t_first, 1e-3)))))
+    t0 = time.perf_counter()
+    for f in range(1, frames):
+        n, _, _ = sc.render_ao(sd.uniforms(W, H, f), params, 0, acc, W, H, nthreads=threads)
+        nrays += n
+    dt = t_first + (time.perf_counter() - t0)
+    prog = ctx.program(solhip.PROG_AO, spec)
+    gacc = ctx.upload(np.zeros((H, W, 4), np.float32))
+    img = solhip.Images(gacc.h, 0, 0, 0, 0, 0)
+    for f in range(frames):
+        ctx.trace(prog, gs.tlas, gs.tables, sd.uniforms(W, H, f), 0, img, W, H)
+    g = gacc.download(np.float32).reshape(H, W, 4)
+    parity = bool(np.array_equal(g.view(np.uint32), acc.view(np.uint32)))
+    return {
+        "value": round(nrays / dt / 1e6, 3),
+        "unit": "Mrays/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "frames 0-%d of the same workload (%d rays, %.1f s wall, %d OpenMP threads); the GPU "
+                  "accumulation of the same frames is bit-identical: %s" % (frames - 1, nrays, dt, threads, parity),
+        "gpu_parity_on_sample": parity,
+    }
+
+
+if __name__ == "__main__":
+    main()

@@ -235,6 +235,27 @@ int32_t solhip_trace(solhip_ctx ctx, solhip_stream s, solhip_program p, solhip_t
 /* counters accumulated since the last reset (rays, fetches, overflows) */
 int32_t solhip_stats(solhip_ctx ctx, solhip_counters *out, uint32_t reset);
 
+/* ---- measurement (no counterpart in the reference, which only reads one
+ * GPU timestamp pair per frame, src/renderer.rs:270-289) ------------------ */
+enum {
+    SOLHIP_KERNEL_TRACE_RAYS = 0,
+    SOLHIP_KERNEL_AO_PRIMARY = 1,
+    SOLHIP_KERNEL_AO_RAYS = 2,
+    SOLHIP_KERNEL_AO_GLSL = 3,
+    SOLHIP_KERNEL_PATHTRACE_GLSL = 4,
+    SOLHIP_KERNEL_DEBUG = 5,
+    SOLHIP_KERNEL_COUNT = 6
+};
+#define SOLHIP_PROFILE_TIME   0x1u  /* hipEvents around every launch, on the launch stream */
+#define SOLHIP_PROFILE_COUNT  0x2u  /* fetch-counting kernel variants */
+int32_t solhip_profile(solhip_ctx ctx, uint32_t flags);
+typedef struct {
+    uint64_t launches;
+    double total_ms;           /* sum of event-measured launch durations */
+    solhip_counters counters;  /* rays / fetches of this kernel */
+} solhip_kernel_stats;
+int32_t solhip_kernel_stats_get(solhip_ctx ctx, uint32_t kernel, solhip_kernel_stats *out, uint32_t reset);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,10 +28,55 @@ struct Obj {
     virtual ~Obj() = default;
 };
 
+// hipEvent pairs around launches, resolved when statistics are read
+struct EventTimer : Timer {
+    struct Rec {
+        int kernel;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> pending;
+    std::vector<Rec> open;
+    void begin(hipStream_t s, int kernel) override
+    {
+        Rec r{kernel, nullptr, nullptr};
+        if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return;
+        (void)hipEventRecord(r.a, s);
+        open.push_back(r);
+    }
+    void end(hipStream_t s, int kernel) override
+    {
+        for (size_t i = open.size(); i-- > 0;) {
+            if (open[i].kernel != kernel) continue;
+            (void)hipEventRecord(open[i].b, s);
+            pending.push_back(open[i]);
+            open.erase(open.begin() + (long)i);
+            return;
+        }
+    }
+};
+
 struct Context : Obj {
     int device = 0;
     hipStream_t stream = nullptr;
-    Counters *counters = nullptr;
+    Counters *counters = nullptr;  // [SOLHIP_KERNEL_COUNT]
+    uint32_t profile = 0;
+    EventTimer timer;
+    uint64_t launches[SOLHIP_KERNEL_COUNT] = {};
+    double total_ms[SOLHIP_KERNEL_COUNT] = {};
+    Timer *timer_or_null() { return (profile & SOLHIP_PROFILE_TIME) ? &timer : nullptr; }
+    void resolve_events()
+    {
+        for (auto &r : timer.pending) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess && r.kernel >= 0 && r.kernel < SOLHIP_KERNEL_COUNT) {
+                launches[r.kernel] += 1;
+                total_ms[r.kernel] += ms;
+            }
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        timer.pending.clear();
+    }
 };
 
 struct Stream : Obj {
@@ -210,12 +255,13 @@ int32_t solhip_context_create(int32_t device, uint32_t flags, solhip_ctx *out)
     Context *c = new Context();
     c->kind = Kind::Context;
     c->device = device;
-    if (hipStreamCreate(&c->stream) != hipSuccess || hipMalloc(&c->counters, sizeof(Counters)) != hipSuccess) {
+    if (hipStreamCreate(&c->stream) != hipSuccess ||
+        hipMalloc(&c->counters, sizeof(Counters) * SOLHIP_KERNEL_COUNT) != hipSuccess) {
         set_error("context allocation failed");
         delete c;
         return SOLHIP_ERROR_DEVICE;
     }
-    (void)hipMemset(c->counters, 0, sizeof(Counters));
+    (void)hipMemset(c->counters, 0, sizeof(Counters) * SOLHIP_KERNEL_COUNT);
     (void)hipDeviceSynchronize();
     const uint64_t h = reg(c);
     c->ctx = h;
@@ -544,7 +590,8 @@ int32_t solhip_trace_rays(solhip_ctx ctx, solhip_stream st, solhip_tlas th, solh
     if (!rb || !hb) return bad_handle("buffer");
     if ((uint64_t)n * sizeof(solhip_ray) > rb->bytes) return bad_arg("ray buffer smaller than n rays");
     if ((uint64_t)n * sizeof(solhip_hit) > hb->bytes) return bad_arg("hit buffer smaller than n hits");
-    return launch_trace_rays(s, view_of(t), (const solhip_ray *)rb->ptr, (solhip_hit *)hb->ptr, n, flags, c->counters);
+    return launch_trace_rays(s, view_of(t), (const solhip_ray *)rb->ptr, (solhip_hit *)hb->ptr, n, flags,
+                             c->counters + SOLHIP_KERNEL_TRACE_RAYS, c->timer_or_null());
 }
 
 void solhip_spec_default(uint32_t kind, solhip_spec *spec)
@@ -674,6 +721,7 @@ int32_t solhip_trace(solhip_ctx ctx, solhip_stream st, solhip_program ph, solhip
     a.spp = p->spec.spp_per_frame;
     a.max_depth = p->spec.max_depth;
     a.counters = c->counters;
+    a.count = (c->profile & SOLHIP_PROFILE_COUNT) != 0;
     if (images->hits_out) {
         Buffer *hb = get<Buffer>(images->hits_out, Kind::Buffer);
         if (!hb) return bad_handle("hits_out buffer");
@@ -688,11 +736,11 @@ int32_t solhip_trace(solhip_ctx ctx, solhip_stream st, solhip_program ph, solhip
         a.qcount = (uint32_t *)p->qcount.ptr;
         int rc = launch_fill(s, a.qcount, 1, 0u);
         if (rc) return rc;
-        return launch_ao(s, a);
+        return launch_ao(s, a, c->timer_or_null());
     }
-    case SOLHIP_PROG_AO_GLSL: return launch_ao_glsl(s, a);
-    case SOLHIP_PROG_PATHTRACE_GLSL: return launch_pathtrace_glsl(s, a);
-    case SOLHIP_PROG_DEBUG: return launch_debug(s, a);
+    case SOLHIP_PROG_AO_GLSL: return launch_ao_glsl(s, a, c->timer_or_null());
+    case SOLHIP_PROG_PATHTRACE_GLSL: return launch_pathtrace_glsl(s, a, c->timer_or_null());
+    case SOLHIP_PROG_DEBUG: return launch_debug(s, a, c->timer_or_null());
     default: return bad_arg("program kind");
     }
 }
@@ -703,15 +751,52 @@ int32_t solhip_stats(solhip_ctx ctx, solhip_counters *out, uint32_t reset)
     if (!c) return bad_handle("context");
     if (!out) return bad_arg("out");
     SOLHIP_HIP_CHECK(hipDeviceSynchronize());
-    Counters h;
-    SOLHIP_HIP_CHECK(hipMemcpy(&h, c->counters, sizeof(h), hipMemcpyDeviceToHost));
-    out->rays = h.rays;
-    out->node_fetches = h.nodes;
-    out->tri_fetches = h.tris;
-    out->inst_fetches = h.insts;
-    out->overflow = h.overflow;
+    Counters h[SOLHIP_KERNEL_COUNT];
+    SOLHIP_HIP_CHECK(hipMemcpy(h, c->counters, sizeof(h), hipMemcpyDeviceToHost));
+    std::memset(out, 0, sizeof(*out));
+    for (int k = 0; k < SOLHIP_KERNEL_COUNT; ++k) {
+        out->rays += h[k].rays;
+        out->node_fetches += h[k].nodes;
+        out->tri_fetches += h[k].tris;
+        out->inst_fetches += h[k].insts;
+        out->overflow += h[k].overflow;
+    }
     if (reset) {
-        SOLHIP_HIP_CHECK(hipMemset(c->counters, 0, sizeof(Counters)));
+        SOLHIP_HIP_CHECK(hipMemset(c->counters, 0, sizeof(h)));
+        SOLHIP_HIP_CHECK(hipDeviceSynchronize());
+    }
+    return SOLHIP_OK;
+}
+
+int32_t solhip_profile(solhip_ctx ctx, uint32_t flags)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    if (flags & ~(SOLHIP_PROFILE_TIME | SOLHIP_PROFILE_COUNT)) return bad_arg("profile flags");
+    c->profile = flags;
+    return SOLHIP_OK;
+}
+
+int32_t solhip_kernel_stats_get(solhip_ctx ctx, uint32_t kernel, solhip_kernel_stats *out, uint32_t reset)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    if (!out || kernel >= SOLHIP_KERNEL_COUNT) return bad_arg("kernel / out");
+    SOLHIP_HIP_CHECK(hipDeviceSynchronize());
+    c->resolve_events();
+    Counters h;
+    SOLHIP_HIP_CHECK(hipMemcpy(&h, c->counters + kernel, sizeof(h), hipMemcpyDeviceToHost));
+    out->launches = c->launches[kernel];
+    out->total_ms = c->total_ms[kernel];
+    out->counters.rays = h.rays;
+    out->counters.node_fetches = h.nodes;
+    out->counters.tri_fetches = h.tris;
+    out->counters.inst_fetches = h.insts;
+    out->counters.overflow = h.overflow;
+    if (reset) {
+        c->launches[kernel] = 0;
+        c->total_ms[kernel] = 0.0;
+        SOLHIP_HIP_CHECK(hipMemset(c->counters + kernel, 0, sizeof(h)));
         SOLHIP_HIP_CHECK(hipDeviceSynchronize());
     }
     return SOLHIP_OK;

@@ -30,6 +30,13 @@ struct alignas(16) AoEntry {
     uint32_t rng;
 };
 
+// launch timing hook (hipEvents on the launch stream), null when disabled
+struct Timer {
+    virtual void begin(hipStream_t s, int kernel) = 0;
+    virtual void end(hipStream_t s, int kernel) = 0;
+    virtual ~Timer() = default;
+};
+
 struct ProgArgs {
     SceneView sv;
     DevTables tb;
@@ -49,15 +56,16 @@ struct ProgArgs {
     uint32_t *qcount;
     // pathtrace
     uint32_t enable_sky, spp, max_depth;
-    Counters *counters;
+    Counters *counters;   // per-kernel counter slots [SOLHIP_KERNEL_COUNT]
+    bool count;           // fetch-counting kernel variants
 };
 
 int launch_trace_rays(hipStream_t s, const SceneView &sv, const solhip_ray *rays, solhip_hit *hits, uint32_t n,
-                      uint32_t flags, Counters *counters);
-int launch_ao(hipStream_t s, const ProgArgs &a);
-int launch_ao_glsl(hipStream_t s, const ProgArgs &a);
-int launch_pathtrace_glsl(hipStream_t s, const ProgArgs &a);
-int launch_debug(hipStream_t s, const ProgArgs &a);
+                      uint32_t flags, Counters *counters, Timer *timer);
+int launch_ao(hipStream_t s, const ProgArgs &a, Timer *timer);
+int launch_ao_glsl(hipStream_t s, const ProgArgs &a, Timer *timer);
+int launch_pathtrace_glsl(hipStream_t s, const ProgArgs &a, Timer *timer);
+int launch_debug(hipStream_t s, const ProgArgs &a, Timer *timer);
 int launch_fill(hipStream_t s, uint32_t *p, uint64_t nwords, uint32_t value);
 
 }  // namespace solhip

@@ -90,14 +90,29 @@ __device__ __forceinline__ void store_hit(solhip_hit *dst, const Hit &h)
     *dst = o;
 }
 
-__device__ __forceinline__ void count_rays(Counters *c, uint32_t rays_this_lane, bool overflow)
+// per-wave sums of rays (and fetches when COUNT), one atomic per wave
+template <bool COUNT>
+__device__ __forceinline__ void count_rays(Counters *c, uint32_t rays_this_lane, const TravCounts &tc)
 {
     unsigned long long r = rays_this_lane;
-    for (int off = 32; off > 0; off >>= 1) r += __shfl_xor(r, off);
-    const uint64_t ov = __ballot(overflow);
+    unsigned long long nd = COUNT ? tc.nodes : 0, tr = COUNT ? tc.tris : 0, in = COUNT ? tc.insts : 0;
+    for (int off = 32; off > 0; off >>= 1) {
+        r += __shfl_xor(r, off);
+        if (COUNT) {
+            nd += __shfl_xor(nd, off);
+            tr += __shfl_xor(tr, off);
+            in += __shfl_xor(in, off);
+        }
+    }
+    const uint64_t ov = __ballot(tc.overflow);
     if ((threadIdx.x & 63) == 0) {
         atomicAdd(&c->rays, r);
         if (ov) atomicAdd(&c->overflow, (unsigned long long)__popcll(ov));
+        if (COUNT) {
+            atomicAdd(&c->nodes, nd);
+            atomicAdd(&c->tris, tr);
+            atomicAdd(&c->insts, in);
+        }
     }
 }
 
@@ -106,6 +121,7 @@ __device__ __forceinline__ float length3(V3 a) { return sqrtf(dot(a, a)); }
 // ---------------------------------------------------------------------------
 // plain ray-traced AO (wavefront)
 // ---------------------------------------------------------------------------
+template <bool COUNT>
 __global__ void __launch_bounds__(kProgBlock) k_ao_primary(ProgArgs A)
 {
     __shared__ uint2 stack[kLdsStack * kProgBlock];
@@ -122,7 +138,7 @@ __global__ void __launch_bounds__(kProgBlock) k_ao_primary(ProgArgs A)
         V3 o, d;
         camera_ray(A.cam.view_inverse, A.cam.projection_inverse, x, y, A.W, A.H, jx, jy, o, d);
         const float tmin = fmaxf(1.0f, length3(o)) * 1e-3f;
-        const Hit h = trace_ray<false, false>(A.sv, o, d, tmin, 10000.0f, 0xFFu, stack + threadIdx.x, kProgBlock, tc);
+        const Hit h = trace_ray<false, COUNT>(A.sv, o, d, tmin, 10000.0f, 0xFFu, stack + threadIdx.x, kProgBlock, tc);
         if (A.hits_out) store_hit(A.hits_out + pix, h);
         if (h.inst != SOLHIP_MISS) {
             const Surf sf = fetch_surface(A.tb, h.inst, h.prim, h.u, h.v);
@@ -167,11 +183,12 @@ __global__ void __launch_bounds__(kProgBlock) k_ao_primary(ProgArgs A)
             A.queue[slot] = e;
         }
     }
-    count_rays(A.counters, active ? 1u : 0u, tc.overflow);
+    count_rays<COUNT>(A.counters + SOLHIP_KERNEL_AO_PRIMARY, active ? 1u : 0u, tc);
 }
 
 // one lane per (queue entry, AO ray); n_ao is a power of two <= 64, so the
 // rays of one pixel are adjacent lanes of one wave
+template <bool COUNT>
 __global__ void __launch_bounds__(kProgBlock) k_ao_rays(ProgArgs A)
 {
     __shared__ uint2 stack[kLdsStack * kProgBlock];
@@ -195,9 +212,9 @@ __global__ void __launch_bounds__(kProgBlock) k_ao_rays(ProgArgs A)
         const V3 o = v3(e.px, e.py, e.pz);
         Hit h;
         if (A.ao_any)
-            h = trace_ray<true, false>(A.sv, o, dir, A.ao_tmin, A.ao_tmax, 0xFFu, stack + threadIdx.x, kProgBlock, tc);
+            h = trace_ray<true, COUNT>(A.sv, o, dir, A.ao_tmin, A.ao_tmax, 0xFFu, stack + threadIdx.x, kProgBlock, tc);
         else
-            h = trace_ray<false, false>(A.sv, o, dir, A.ao_tmin, A.ao_tmax, 0xFFu, stack + threadIdx.x, kProgBlock, tc);
+            h = trace_ray<false, COUNT>(A.sv, o, dir, A.ao_tmin, A.ao_tmax, 0xFFu, stack + threadIdx.x, kProgBlock, tc);
         occl = h.inst != SOLHIP_MISS;
         if (A.hits_out) store_hit(A.hits_out + (uint64_t)A.W * A.H + (uint64_t)pix * A.n_ao + k, h);
     }
@@ -212,26 +229,43 @@ __global__ void __launch_bounds__(kProgBlock) k_ao_rays(ProgArgs A)
         const float4 old = *px;
         *px = make_float4(mixf(old.x, value, A.alpha), mixf(old.y, value, A.alpha), mixf(old.z, value, A.alpha), 1.0f);
     }
-    count_rays(A.counters, active ? 1u : 0u, tc.overflow);
+    count_rays<COUNT>(A.counters + SOLHIP_KERNEL_AO_RAYS, active ? 1u : 0u, tc);
 }
 
-int launch_ao(hipStream_t s, const ProgArgs &a)
+static int launch_status(const char *what)
 {
-    const uint32_t tiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
-    hipLaunchKernelGGL(k_ao_primary, dim3(tiles), dim3(kProgBlock), 0, s, a);
-    const uint64_t lanes = (uint64_t)a.W * a.H * a.n_ao;
-    hipLaunchKernelGGL(k_ao_rays, dim3((uint32_t)((lanes + kProgBlock - 1) / kProgBlock)), dim3(kProgBlock), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
-        set_error(std::string("AO launch: ") + hipGetErrorString(e));
+        set_error(std::string(what) + " launch: " + hipGetErrorString(e));
         return SOLHIP_ERROR_DEVICE;
     }
     return 0;
 }
 
+#define SOLHIP_LAUNCH(kernel, id, grid, a, s, timer)                                            \
+    do {                                                                                      \
+        if (timer) (timer)->begin(s, id);                                                     \
+        if ((a).count)                                                                        \
+            hipLaunchKernelGGL(kernel<true>, dim3(grid), dim3(kProgBlock), 0, s, a);          \
+        else                                                                                  \
+            hipLaunchKernelGGL(kernel<false>, dim3(grid), dim3(kProgBlock), 0, s, a);         \
+        if (timer) (timer)->end(s, id);                                                       \
+    } while (0)
+
+int launch_ao(hipStream_t s, const ProgArgs &a, Timer *timer)
+{
+    const uint32_t tiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
+    SOLHIP_LAUNCH(k_ao_primary, SOLHIP_KERNEL_AO_PRIMARY, tiles, a, s, timer);
+    const uint64_t lanes = (uint64_t)a.W * a.H * a.n_ao;
+    const uint32_t blocks = (uint32_t)((lanes + kProgBlock - 1) / kProgBlock);
+    SOLHIP_LAUNCH(k_ao_rays, SOLHIP_KERNEL_AO_RAYS, blocks, a, s, timer);
+    return launch_status("AO");
+}
+
 // ---------------------------------------------------------------------------
 // ao.rgen / ao.rchit / ao.rmiss
 // ---------------------------------------------------------------------------
+template <bool COUNT>
 __global__ void __launch_bounds__(kProgBlock) k_ao_glsl(ProgArgs A)
 {
     __shared__ uint2 stack[kLdsStack * kProgBlock];
@@ -253,7 +287,7 @@ __global__ void __launch_bounds__(kProgBlock) k_ao_glsl(ProgArgs A)
             const uint32_t sampleId = (uint32_t)i;
             float rmin = fmaxf(1.0f, length3(o)) * 1e-3f, rmax = 10000.0f;
             for (;;) {
-                const Hit h = trace_ray<false, false>(A.sv, o, d, rmin, rmax, 0xFFu, stack + threadIdx.x,
+                const Hit h = trace_ray<false, COUNT>(A.sv, o, d, rmin, rmax, 0xFFu, stack + threadIdx.x,
                                                       kProgBlock, tc);
                 ++rays;
                 if (h.inst == SOLHIP_MISS) {
@@ -293,19 +327,14 @@ __global__ void __launch_bounds__(kProgBlock) k_ao_glsl(ProgArgs A)
         const float4 old = *px;
         *px = make_float4(mixf(old.x, color, A.alpha), mixf(old.y, color, A.alpha), mixf(old.z, color, A.alpha), 1.0f);
     }
-    count_rays(A.counters, rays, tc.overflow);
+    count_rays<COUNT>(A.counters + SOLHIP_KERNEL_AO_GLSL, rays, tc);
 }
 
-int launch_ao_glsl(hipStream_t s, const ProgArgs &a)
+int launch_ao_glsl(hipStream_t s, const ProgArgs &a, Timer *timer)
 {
     const uint32_t tiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
-    hipLaunchKernelGGL(k_ao_glsl, dim3(tiles), dim3(kProgBlock), 0, s, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) {
-        set_error(std::string("AO-glsl launch: ") + hipGetErrorString(e));
-        return SOLHIP_ERROR_DEVICE;
-    }
-    return 0;
+    SOLHIP_LAUNCH(k_ao_glsl, SOLHIP_KERNEL_AO_GLSL, tiles, a, s, timer);
+    return launch_status("AO-glsl");
 }
 
 // ---------------------------------------------------------------------------
@@ -327,6 +356,7 @@ __device__ __forceinline__ V3 pt_miss(uint32_t enable_sky, V3 d)
     return v3(0.0f, 0.0f, 0.0f);
 }
 
+template <bool COUNT>
 __global__ void __launch_bounds__(kProgBlock) k_pathtrace_glsl(ProgArgs A)
 {
     __shared__ uint2 stack[kLdsStack * kProgBlock];
@@ -347,7 +377,7 @@ __global__ void __launch_bounds__(kProgBlock) k_pathtrace_glsl(ProgArgs A)
             uint32_t depth = 0, done = 0;
             float acc[3] = {1.0f, 1.0f, 1.0f};
             for (;;) {
-                const Hit h = trace_ray<false, false>(A.sv, o, d, rmin, rmax, 0xFFu, stack + threadIdx.x,
+                const Hit h = trace_ray<false, COUNT>(A.sv, o, d, rmin, rmax, 0xFFu, stack + threadIdx.x,
                                                       kProgBlock, tc);
                 ++rays;
                 V3 hv;
@@ -424,24 +454,20 @@ __global__ void __launch_bounds__(kProgBlock) k_pathtrace_glsl(ProgArgs A)
             A.out_rgba8[(uint64_t)y * A.W + x] = r8 | (g8 << 8) | (b8 << 16) | (255u << 24);
         }
     }
-    count_rays(A.counters, rays, tc.overflow);
+    count_rays<COUNT>(A.counters + SOLHIP_KERNEL_PATHTRACE_GLSL, rays, tc);
 }
 
-int launch_pathtrace_glsl(hipStream_t s, const ProgArgs &a)
+int launch_pathtrace_glsl(hipStream_t s, const ProgArgs &a, Timer *timer)
 {
     const uint32_t tiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
-    hipLaunchKernelGGL(k_pathtrace_glsl, dim3(tiles), dim3(kProgBlock), 0, s, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) {
-        set_error(std::string("pathtrace launch: ") + hipGetErrorString(e));
-        return SOLHIP_ERROR_DEVICE;
-    }
-    return 0;
+    SOLHIP_LAUNCH(k_pathtrace_glsl, SOLHIP_KERNEL_PATHTRACE_GLSL, tiles, a, s, timer);
+    return launch_status("pathtrace");
 }
 
 // ---------------------------------------------------------------------------
 // debug.rgen / debug.rchit / debug.rmiss
 // ---------------------------------------------------------------------------
+template <bool COUNT>
 __global__ void __launch_bounds__(kProgBlock) k_debug(ProgArgs A)
 {
     __shared__ uint2 stack[kLdsStack * kProgBlock];
@@ -451,7 +477,7 @@ __global__ void __launch_bounds__(kProgBlock) k_debug(ProgArgs A)
     if (active) {
         V3 o, d;
         camera_ray(A.cam.view_inverse, A.cam.projection_inverse, x, y, A.W, A.H, 0.5f, 0.5f, o, d);  // debug.rgen:20-26
-        const Hit h = trace_ray<false, false>(A.sv, o, d, 0.001f, 1000.0f, 0xFFu, stack + threadIdx.x, kProgBlock, tc);
+        const Hit h = trace_ray<false, COUNT>(A.sv, o, d, 0.001f, 1000.0f, 0xFFu, stack + threadIdx.x, kProgBlock, tc);
         float4 r;
         if (h.inst == SOLHIP_MISS)
             r = make_float4(d.x, d.y, d.z, 0.0f);                       // debug.rgen:28
@@ -459,19 +485,14 @@ __global__ void __launch_bounds__(kProgBlock) k_debug(ProgArgs A)
             r = make_float4(1.0f - h.u - h.v, h.u, h.v, 0.0f);          // debug.rchit:11-12
         A.accum[(uint64_t)y * A.W + x] = r;
     }
-    count_rays(A.counters, active ? 1u : 0u, tc.overflow);
+    count_rays<COUNT>(A.counters + SOLHIP_KERNEL_DEBUG, active ? 1u : 0u, tc);
 }
 
-int launch_debug(hipStream_t s, const ProgArgs &a)
+int launch_debug(hipStream_t s, const ProgArgs &a, Timer *timer)
 {
     const uint32_t tiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
-    hipLaunchKernelGGL(k_debug, dim3(tiles), dim3(kProgBlock), 0, s, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) {
-        set_error(std::string("debug launch: ") + hipGetErrorString(e));
-        return SOLHIP_ERROR_DEVICE;
-    }
-    return 0;
+    SOLHIP_LAUNCH(k_debug, SOLHIP_KERNEL_DEBUG, tiles, a, s, timer);
+    return launch_status("debug");
 }
 
 }  // namespace solhip

@@ -66,9 +66,10 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_rays(SceneView sv, const 
 }
 
 int launch_trace_rays(hipStream_t s, const SceneView &sv, const solhip_ray *rays, solhip_hit *hits, uint32_t n,
-                      uint32_t flags, Counters *counters)
+                      uint32_t flags, Counters *counters, Timer *timer)
 {
     if (n == 0) return 0;
+    if (timer) timer->begin(s, SOLHIP_KERNEL_TRACE_RAYS);
     const dim3 grid((n + kTraceBlock - 1) / kTraceBlock), block(kTraceBlock);
     const bool any = flags & SOLHIP_TRACE_TERMINATE_ON_FIRST_HIT, count = flags & SOLHIP_TRACE_COUNT;
     if (any && count)
@@ -79,6 +80,7 @@ int launch_trace_rays(hipStream_t s, const SceneView &sv, const solhip_ray *rays
         hipLaunchKernelGGL((k_trace_rays<false, true>), grid, block, 0, s, sv, rays, hits, n, counters);
     else
         hipLaunchKernelGGL((k_trace_rays<false, false>), grid, block, 0, s, sv, rays, hits, n, counters);
+    if (timer) timer->end(s, SOLHIP_KERNEL_TRACE_RAYS);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         set_error(std::string("trace launch: ") + hipGetErrorString(e));

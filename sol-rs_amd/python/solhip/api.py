@@ -56,6 +56,14 @@ class Counters(C.Structure):
                 ("inst_fetches", C.c_uint64), ("overflow", C.c_uint64)]
 
 
+class KernelStats(C.Structure):
+    _fields_ = [("launches", C.c_uint64), ("total_ms", C.c_double), ("counters", Counters)]
+
+
+KERNELS = {"trace_rays": 0, "ao_primary": 1, "ao_rays": 2, "ao_glsl": 3, "pathtrace_glsl": 4, "debug": 5}
+PROFILE_TIME, PROFILE_COUNT = 0x1, 0x2
+
+
 class BvhInfo(C.Structure):
     _fields_ = [("nodes", C.c_uint32), ("prims", C.c_uint32), ("depth", C.c_uint32), ("leaf_max", C.c_uint32),
                 ("bytes", C.c_uint64)]
@@ -91,6 +99,8 @@ SIGNATURES = {
     "solhip_program_destroy": (_i32, [_u64, _u64]),
     "solhip_trace": (_i32, [_u64, _u64, _u64, _u64, C.POINTER(SceneTables), _vp, _i32, C.POINTER(Images), _u32, _u32]),
     "solhip_stats": (_i32, [_u64, C.POINTER(Counters), _u32]),
+    "solhip_profile": (_i32, [_u64, _u32]),
+    "solhip_kernel_stats_get": (_i32, [_u64, _u32, C.POINTER(KernelStats), _u32]),
 }
 
 _lib = None
@@ -262,6 +272,15 @@ class Context:
         c = Counters()
         check(lib().solhip_stats(self.h, C.byref(c), 1 if reset else 0))
         return c
+
+    def profile(self, flags):
+        check(lib().solhip_profile(self.h, flags))
+
+    def kernel_stats(self, kernel, reset=False):
+        k = KERNELS[kernel] if isinstance(kernel, str) else kernel
+        st = KernelStats()
+        check(lib().solhip_kernel_stats_get(self.h, k, C.byref(st), 1 if reset else 0))
+        return st
 
 
 class GpuScene:

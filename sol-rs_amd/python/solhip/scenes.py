@@ -301,9 +301,9 @@ def heightfield(n=1024, seed=1, amplitude=0.2, W=1920, H=1080):
     idx = np.stack([a, c, b, b, c, d], axis=-1).reshape(-1).astype(np.uint32)
     blas = [Blas(v, idx)]
     el = math.radians(35.0)
-    dist = 2.4
+    dist = 1.5
     cam = Camera((0.0, dist * math.sin(el), dist * math.cos(el)), (0.0, 0.0, 0.0), up=(0.0, -1.0, 0.0),
-                 vfov_deg=50.0)
+                 vfov_deg=40.0)
     mats = [material((0.8, 0.8, 0.8))]
     return assemble("heightfield_%d" % n, blas, [0], [np.eye(4)], [0], mats, cam, 2.0)
 

@@ -1,0 +1,272 @@
+"""Pins the CPU oracle (oracle/solref.c) against the golden vectors written by an
+independent restatement, closed forms and brute force.  CPU only."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import raygen
+from solhip import scenes
+from solhip.records import RAY
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+# ---- RNG: sampling.glsl:18-47 -------------------------------------------------
+def test_tea_and_pcg_match_golden_vectors(oracle):
+    g = np.load(os.path.join(HERE, "golden", "rng_golden.npz"))
+    for p, f, s, fl in zip(g["pixel"], g["frames"], g["seeds"], g["floats"]):
+        assert oracle.tea(int(p), int(f)) == int(s)
+        out, _ = oracle.next_rand_stream(int(s), fl.shape[0])
+        assert out.view(np.uint32).tolist() == fl.view(np.uint32).tolist()
+    for st, fl in zip(g["edge_states"], g["edge_floats"]):
+        out, _ = oracle.next_rand_stream(int(st), fl.shape[0])
+        assert out.view(np.uint32).tolist() == fl.view(np.uint32).tolist()
+
+
+def test_tea_known_answer(oracle):
+    # TEA-16 of (0, 0): fixed by the algorithm alone
+    from golden.make_golden import tea
+    assert oracle.tea(0, 0) == int(tea(np.array([0], np.uint32), np.array([0], np.uint32))[0])
+    assert oracle.tea(0, 0) != oracle.tea(1, 0) != oracle.tea(0, 1)
+
+
+def test_next_rand_is_word_times_2_pow_minus_32(oracle):
+    # the output can reach exactly 1.0 (word rounds to 2^32) and is never > 1
+    out, _ = oracle.next_rand_stream(12345, 20000)
+    assert out.min() >= 0.0 and out.max() <= 1.0
+    assert abs(out.mean() - 0.5) < 0.01
+
+
+def test_pcg_advance_equals_repeated_steps(oracle):
+    for seed in (0, 1, 0xDEADBEEF, 2 ** 31):
+        _, st = oracle.next_rand_stream(seed, 37)
+        assert oracle.pcg_advance(seed, 37) == st
+        assert oracle.pcg_advance(seed, 0) == seed
+
+
+# ---- own sin/cos against libm (double) ------------------------------------------
+def test_sincos_within_2_ulp_on_sampling_range(oracle):
+    xs = np.linspace(0.0, 2 * math.pi, 20001, dtype=np.float32)
+    for x in xs[::7]:
+        s = oracle.sinf(float(x))
+        c = oracle.cosf(float(x))
+        rs, rc = math.sin(float(x)), math.cos(float(x))
+        ulp = np.spacing(np.float32(1.0))
+        assert abs(s - rs) <= 2 * ulp * max(1.0, abs(rs)) + 1e-7
+        assert abs(c - rc) <= 2 * ulp * max(1.0, abs(rc)) + 1e-7
+
+
+# ---- sampling closed forms --------------------------------------------------------
+def test_onb_is_orthonormal(oracle):
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        n = rng.normal(size=3)
+        n = (n / np.linalg.norm(n)).astype(np.float32)
+        u, v = oracle.onb(n)
+        M = np.stack([u, v, n]).astype(np.float64)
+        assert np.allclose(M @ M.T, np.eye(3), atol=2e-6)
+
+
+def test_cosine_lobe_moments(oracle):
+    # unit length; E[cos theta] = 2/3 and E[cos^2] = 1/2 for the cosine lobe
+    rng = np.random.default_rng(1)
+    n = np.array([0.3, -0.5, 0.8], np.float32)
+    n /= np.linalg.norm(n)
+    cs = []
+    for xi in rng.random((20000, 2), dtype=np.float32):
+        d = oracle.sample_cosine(n, xi)
+        assert abs(np.linalg.norm(d) - 1.0) < 2e-6
+        cs.append(float(d @ n))
+    cs = np.array(cs)
+    assert cs.min() >= -1e-6
+    assert abs(cs.mean() - 2 / 3) < 0.01 and abs((cs ** 2).mean() - 0.5) < 0.01
+
+
+def test_ggx_with_zero_roughness_returns_the_axis(oracle):
+    rng = np.random.default_rng(2)
+    n = np.array([0.0, 0.6, 0.8], np.float32)
+    for xi in rng.random((100, 2), dtype=np.float32) * np.float32(0.999):
+        assert np.array_equal(oracle.sample_ggx(n, xi, 0.0), n)
+
+
+def test_ggx_alpha_one_is_cosine_squared_uniform(oracle):
+    # alpha^2 = 1: cos^2(theta) = 1 - xi, uniform in [0, 1]
+    rng = np.random.default_rng(3)
+    n = np.array([0.0, 0.0, 1.0], np.float32)
+    c2 = [float(oracle.sample_ggx(n, xi, 1.0)[2]) ** 2 for xi in rng.random((10000, 2), dtype=np.float32)]
+    assert abs(np.mean(c2) - 0.5) < 0.01
+
+
+def test_fresnel_normal_and_grazing(oracle):
+    n = np.array([0.0, 0.0, 1.0], np.float32)
+    assert abs(oracle.fresnel(n, n, 1.0 / 1.5) - 0.04) < 1e-6
+    g = np.array([1.0, 0.0, 0.0], np.float32)
+    assert abs(oracle.fresnel(n, g, 1.0 / 1.5) - 1.0) < 1e-6
+
+
+# ---- camera --------------------------------------------------------------------------
+def test_centre_ray_is_the_view_axis(oracle):
+    cam = scenes.Camera((1.0, 2.0, 3.0), (0.0, 0.5, -1.0), up=(0.0, -1.0, 0.0), vfov_deg=40.0)
+    u = cam.uniforms(64, 64, 0)
+    o, d = oracle.camera_ray(u, 32, 32, 64, 64, 0.0, 0.0)
+    axis = np.array([0.0, 0.5, -1.0]) - np.array([1.0, 2.0, 3.0])
+    axis /= np.linalg.norm(axis)
+    assert np.allclose(o, [1, 2, 3], atol=1e-6)
+    assert np.allclose(d, axis, atol=1e-6)
+
+
+def test_image_corners_span_the_vertical_fov(oracle):
+    cam = scenes.Camera((0.0, 0.0, 5.0), (0.0, 0.0, 0.0), up=(0.0, -1.0, 0.0), vfov_deg=40.0)
+    u = cam.uniforms(100, 100, 0)
+    _, top = oracle.camera_ray(u, 50, 0, 100, 100, 0.0, 0.0)
+    _, bot = oracle.camera_ray(u, 50, 99, 100, 100, 0.0, 1.0)
+    ang = math.degrees(math.acos(float(np.clip(top @ bot, -1, 1))))
+    assert abs(ang - 40.0) < 0.05
+    assert top[1] > 0 > bot[1]   # row 0 is the top of the image (up = -Y, Vulkan clip space)
+
+
+# ---- watertight triangle test ----------------------------------------------------------
+def test_axis_aligned_quad_exact_t(oracle):
+    v0, v1, v2 = [0, 0, 0], [1, 0, 0], [0, 1, 0]
+    hit, t, u, v = oracle.intersect_triangle([0.25, 0.25, 2.0], [0, 0, -1], 0.0, 10.0, v0, v1, v2)
+    assert hit and t == 2.0 and u == 0.25 and v == 0.25
+    hit, *_ = oracle.intersect_triangle([0.25, 0.25, 2.0], [0, 0, -1], 0.0, 1.5, v0, v1, v2)
+    assert not hit   # beyond tmax
+    hit, t, *_ = oracle.intersect_triangle([0.25, 0.25, -2.0], [0, 0, 1], 0.0, 10.0, v0, v1, v2)
+    assert hit and t == 2.0   # back face: no culling
+
+
+def test_shared_edge_is_watertight(oracle):
+    # two triangles sharing the edge (1,0)-(0,1); rays through points of the
+    # edge hit at least one of them
+    a = ([0, 0, 0], [1, 0, 0], [0, 1, 0])
+    b = ([1, 0, 0], [1, 1, 0], [0, 1, 0])
+    rng = np.random.default_rng(4)
+    for s in rng.random(2000):
+        p = np.array([1 - s, s, 0.0], np.float32)
+        o = p + np.array([0.3, -0.2, 1.0], np.float32)
+        d = p - o
+        ha = oracle.intersect_triangle(o, d, 0.0, 10.0, *a)[0]
+        hb = oracle.intersect_triangle(o, d, 0.0, 10.0, *b)[0]
+        assert ha or hb
+
+
+def test_instance_inverse_round_trip(oracle):
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        m = np.eye(4)
+        m[:3, :3] = rng.normal(size=(3, 3))
+        m[:3, 3] = rng.normal(size=3)
+        inv = oracle.instance_inverse(m[:3, :4].reshape(-1).astype(np.float32)).reshape(3, 4)
+        M = np.vstack([inv, [0, 0, 0, 1]]) @ m
+        assert np.allclose(M, np.eye(4), atol=1e-4)
+
+
+# ---- closest hit: BVH == brute force ---------------------------------------------------
+@pytest.mark.parametrize("make", [scenes.cornell_box, lambda: scenes.random_soup(1500, seed=1),
+                                  scenes.random_instances, lambda: scenes.heightfield(32)])
+def test_oracle_bvh_equals_brute_force(oracle, make):
+    sd = make()
+    sc = oracle.Scene(sd)
+    rays = np.concatenate([raygen.random_rays(2000, (0, 0, 0), 3.0, seed=7), raygen.aimed_rays(sd, 1500, seed=8),
+                           raygen.axis_rays(500)])
+    a = sc.trace(rays, use_bvh=False)
+    b = sc.trace(rays, use_bvh=True)
+    assert a.tobytes() == b.tobytes()
+
+
+def test_tie_rule_prefers_lower_instance_then_primitive(oracle):
+    # two instances of the same triangle at the same place: equal t, the lower
+    # instance index wins; duplicated triangles in one BLAS: the lower prim wins
+    tri = [(-1.0, -1.0, 0.0), (1.0, -1.0, 0.0), (0.0, 1.0, 0.0)]
+    b = scenes.soup_blas(tri + tri)
+    sd = scenes.assemble("dup", [b], [0, 0], [np.eye(4), np.eye(4)], [0, 0], [scenes.material((1, 1, 1))],
+                         scenes.Camera((0, 0, 3), (0, 0, 0)), 2.0)
+    r = np.zeros(1, RAY)
+    r["o"] = (0.0, 0.0, 2.0)
+    r["d"] = (0.0, 0.0, -1.0)
+    r["tmax"] = 10.0
+    for bvh in (False, True):
+        h = oracle.Scene(sd).trace(r, use_bvh=bvh)[0]
+        assert (h["inst"], h["prim"], h["t"]) == (0, 0, 2.0)
+
+
+# ---- programs: closed forms ---------------------------------------------------------------
+def _plane(size=20.0, y=0.0):
+    s = size
+    return [(-s, y, -s), (s, y, -s), (s, y, s), (-s, y, -s), (s, y, s), (-s, y, s)]
+
+
+def test_ao_of_an_open_plane_is_one(oracle):
+    b = scenes.soup_blas(_plane())
+    sd = scenes.assemble("plane", [b], [0], [np.eye(4)], [0], [scenes.material((1, 1, 1))],
+                         scenes.Camera((0, 2, 2), (0, 0, 0)), 2.0)
+    W = H = 24
+    acc = np.zeros((H, W, 4), np.float32)
+    oracle.Scene(sd).render_ao(sd.uniforms(W, H, 0), oracle.AOParams(8, 1e-4, 0.2, 0), 0, acc, W, H)
+    assert (acc[..., :3] == 1.0).all()
+
+
+def test_ao_under_a_parallel_ceiling_matches_closed_form(oracle):
+    # cosine rays from the floor hit a ceiling at height h within range R
+    # iff cos(theta) >= h / R: P(occluded) = 1 - (h / R)^2
+    h, R = 0.1, 0.2
+    floor = scenes.soup_blas(_plane(30.0, 0.0))
+    ceil = scenes.soup_blas(_plane(30.0, h))
+    sd = scenes.assemble("slab", [floor, ceil], [0, 1], [np.eye(4)] * 2, [0, 0], [scenes.material((1, 1, 1))],
+                         scenes.Camera((0.0, 0.05, 0.0), (0.0, 0.0, -1.0)), 2.0)
+    W, H = 64, 64
+    u = sd.uniforms(W, H, 0)
+    acc = np.zeros((H, W, 4), np.float32)
+    n, ph, ah = oracle.Scene(sd).render_ao(u, oracle.AOParams(8, 1e-5, R, 0), 0, acc, W, H, want_hits=True)
+    on_floor = ph["inst"] == 0
+    assert on_floor.sum() > 500
+    occl = 1.0 - acc[..., 0].reshape(-1)[on_floor]
+    expect = 1.0 - (h / R) ** 2
+    assert abs(occl.mean() - expect) < 0.02
+
+
+def test_pathtrace_sky_closed_form(oracle):
+    # empty scene, sky on: every path is one miss; looking straight up the
+    # gradient is at its top colour (0.3, 0.4, 0.5), straight down at its
+    # bottom colour (0.58, 0.45, 0.25)
+    sd = scenes.empty_scene()
+    sc = oracle.Scene(sd)
+    for eye, col in (((0, 0, 0), (0.3, 0.4, 0.5)), ((0, 0, 0), (0.58, 0.45, 0.25))):
+        target = (0.0, 1.0, 0.001) if col[0] == 0.3 else (0.0, -1.0, 0.001)
+        sd.camera = scenes.Camera(eye, target, up=(0.0, 0.0, -1.0), vfov_deg=1.0)
+        W = H = 4
+        acc = np.zeros((H, W, 4), np.float32)
+        sc.render_pathtrace_glsl(sd.uniforms(W, H, 0), oracle.PTParams(1, 8, 32), 0, acc, None, W, H)
+        assert np.allclose(acc[..., :3], col, atol=1e-4)
+
+
+def test_pathtrace_sees_emitter_radiance(oracle):
+    # the camera looks straight into a large emissive quad: radiance 15
+    quad = [(-5, -5, 0), (5, -5, 0), (5, 5, 0), (-5, -5, 0), (5, 5, 0), (-5, 5, 0)]
+    b = scenes.soup_blas(quad)
+    sd = scenes.assemble("light", [b], [0], [np.eye(4)], [0],
+                         [scenes.material((1, 1, 1), emissive=(15, 15, 15))], scenes.Camera((0, 0, 2), (0, 0, 0)),
+                         2.0)
+    W = H = 8
+    acc = np.zeros((H, W, 4), np.float32)
+    out = np.zeros((H, W, 4), np.uint8)
+    oracle.Scene(sd).render_pathtrace_glsl(sd.uniforms(W, H, 0), oracle.PTParams(0, 8, 32), 0, acc, out, W, H)
+    assert (acc[..., :3] == 15.0).all() and (out[..., :3] == 255).all()
+
+
+def test_accumulation_is_a_running_mean(oracle):
+    # frames f = 0..3 with accum_start 0: value = mean of the four frame values
+    sd = scenes.cornell_box()
+    sc = oracle.Scene(sd)
+    W = H = 8
+    acc = np.zeros((H, W, 4), np.float32)
+    singles = []
+    for f in range(4):
+        one = np.zeros((H, W, 4), np.float32)
+        sc.render_ao(sd.uniforms(W, H, f), oracle.AOParams(8, 2e-4, 0.2, 0), f, one, W, H)
+        singles.append(one[..., 0].astype(np.float64))
+        sc.render_ao(sd.uniforms(W, H, f), oracle.AOParams(8, 2e-4, 0.2, 0), 0, acc, W, H)
+    assert np.allclose(acc[..., 0], np.mean(singles, axis=0), atol=1e-6)
