@@ -54,6 +54,8 @@ def parse_args():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", type=int, default=0)
+    ap.add_argument("--any-hit", action="store_true",
+                    help="AO rays stop at the first hit (same image; hit records are not the closest)")
     return ap.parse_args()
 
 
@@ -117,7 +119,7 @@ def main():
     spec.n_ao = 8
     spec.ao_tmin = 1e-4 * sd.extent
     spec.ao_tmax = 0.1 * sd.extent
-    spec.ao_any_hit = 0
+    spec.ao_any_hit = 1 if a.any_hit else 0
     prog = ctx.program(solhip.PROG_AO, spec)
     acc = ctx.upload(np.zeros((H, W, 4), np.float32))
     img = solhip.Images(acc.h, 0, 0, 0, 0, 0)
@@ -177,8 +179,9 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "plain ray-traced AO, %dx%d-quad fBm heightfield (%d triangles, one BLAS), %dx%d, "
-                        "1 primary + %d cosine AO rays per primary hit, closest hit, AO range [%.0e, %.2f]"
-                        % (a.grid, a.grid, sd.tri_count, W, H, spec.n_ao, spec.ao_tmin, spec.ao_tmax),
+                        "1 primary + %d cosine AO rays per primary hit, %s, AO range [%.0e, %.2f]"
+                        % (a.grid, a.grid, sd.tri_count, W, H, spec.n_ao,
+                           "first hit" if a.any_hit else "closest hit", spec.ao_tmin, spec.ao_tmax),
             "triangles": sd.tri_count,
             "resolution": [W, H],
             "n_ao": spec.n_ao,

@@ -5,6 +5,7 @@
 // Buffer::from_data / Buffer::update of the reference (src/buffer.rs:144-225).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -14,6 +15,7 @@
 #include "bvh.h"
 #include "common.h"
 #include "kernels.h"
+#include "traverse.h"
 
 namespace solhip {
 
@@ -98,6 +100,7 @@ struct Tlas : Obj {
     DevScratch scratch;
     DevBuf inst_dev;   // solhip_instance[]
     DevBuf refs_dev;   // BlasRef[]
+    std::vector<uint64_t> blas_handles;   // distinct BLAS referenced by the instances
 };
 
 struct Program : Obj {
@@ -168,9 +171,22 @@ hipStream_t stream_of(Context *cx, solhip_stream s, bool &ok)
     return st->s;
 }
 
+// LDS stack entries a traversal of this TLAS needs: one per TLAS level plus
+// one per level of the deepest referenced BLAS (plus one spare)
+uint32_t stack_depth_of(const Tlas *t)
+{
+    uint32_t bd = 0;
+    for (uint64_t h : t->blas_handles) {
+        const Blas *b = get<Blas>(h, Kind::Blas);
+        if (b) bd = std::max(bd, b->bvh.depth);
+    }
+    return t->bvh.depth + bd + 1;
+}
+
 SceneView view_of(const Tlas *t)
 {
     SceneView v;
+    v.stack_depth = std::min<uint32_t>(stack_depth_of(t), (uint32_t)kMaxStack);
     if (t->bvh.prim_count == 0) {
         v.tnodes = nullptr;
         v.insts = nullptr;
@@ -186,8 +202,11 @@ SceneView view_of(const Tlas *t)
 int tlas_build_into(Context *cx, hipStream_t s, Tlas *t, const solhip_instance *inst, uint32_t n)
 {
     std::vector<BlasRef> refs(n);
+    t->blas_handles.clear();
     for (uint32_t i = 0; i < n; ++i) {
         Blas *b = get<Blas>(inst[i].blas, Kind::Blas);
+        if (b && std::find(t->blas_handles.begin(), t->blas_handles.end(), inst[i].blas) == t->blas_handles.end())
+            t->blas_handles.push_back(inst[i].blas);
         if (!b) {
             set_error("instance " + std::to_string(i) + " references an invalid BLAS handle");
             return SOLHIP_ERROR_INVALID_HANDLE;
@@ -590,6 +609,10 @@ int32_t solhip_trace_rays(solhip_ctx ctx, solhip_stream st, solhip_tlas th, solh
     if (!rb || !hb) return bad_handle("buffer");
     if ((uint64_t)n * sizeof(solhip_ray) > rb->bytes) return bad_arg("ray buffer smaller than n rays");
     if ((uint64_t)n * sizeof(solhip_hit) > hb->bytes) return bad_arg("hit buffer smaller than n hits");
+    if (stack_depth_of(t) > (uint32_t)kMaxStack) {
+        set_error("acceleration structures too deep for the traversal stack");
+        return SOLHIP_ERROR_UNSUPPORTED;
+    }
     return launch_trace_rays(s, view_of(t), (const solhip_ray *)rb->ptr, (solhip_hit *)hb->ptr, n, flags,
                              c->counters + SOLHIP_KERNEL_TRACE_RAYS, c->timer_or_null());
 }
@@ -676,6 +699,10 @@ int32_t solhip_trace(solhip_ctx ctx, solhip_stream st, solhip_program ph, solhip
         if (tables->index_size != 4 && tables->index_size != 8) return bad_arg("index_size must be 4 or 8");
         if (p->kind_id == SOLHIP_PROG_PATHTRACE_GLSL && !tables->material_table_addr)
             return bad_arg("pathtrace needs the material table");
+    }
+    if (stack_depth_of(t) > (uint32_t)kMaxStack) {
+        set_error("acceleration structures too deep for the traversal stack");
+        return SOLHIP_ERROR_UNSUPPORTED;
     }
     ProgArgs a;
     std::memset(&a, 0, sizeof(a));

@@ -70,6 +70,7 @@ struct SceneView {
     const WideNode *tnodes;     // TLAS nodes (nullptr: empty TLAS)
     const InstRec *insts;       // TLAS leaf records
     const uint32_t *inst_meta;  // index | mask << 24
+    uint32_t stack_depth;       // LDS stack entries per lane (>= TLAS + BLAS depth)
 };
 
 struct Counters {

@@ -18,7 +18,17 @@
 
 namespace solhip {
 
-constexpr int kProgBlock = 64;
+#ifndef SOLHIP_PROG_BLOCK
+#define SOLHIP_PROG_BLOCK 64
+#endif
+constexpr int kProgBlock = SOLHIP_PROG_BLOCK;  // a multiple of 64
+// minimum waves per SIMD requested for the AO ray kernel: it is VALU-issue
+// bound, so the register budget is traded against latency hiding
+#ifndef SOLHIP_AO_WAVES
+#define SOLHIP_AO_WAVES 4
+#endif
+#define SOLHIP_AO_RAYS_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SOLHIP_AO_WAVES)))
+constexpr int kTileH = kProgBlock / 8;          // pixel tiles are 8 x kTileH
 
 __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
 {
@@ -26,16 +36,19 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
     return x * q + min(x, r) + (b >> 3);
 }
 
-// pixel of this lane: 8x8 tiles, tiles in row-major order, XCD-contiguous
+// pixel of this lane: 8 x kTileH tiles (one wave = 8x8 pixels), tiles in
+// row-major order, XCD-contiguous
 __device__ __forceinline__ bool tile_pixel(uint32_t W, uint32_t H, uint32_t &x, uint32_t &y)
 {
     const uint32_t tiles_x = (W + 7) / 8;
     const uint32_t t = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t tx = t % tiles_x, ty = t / tiles_x;
     x = tx * 8 + (threadIdx.x & 7);
-    y = ty * 8 + (threadIdx.x >> 3);
+    y = ty * kTileH + (threadIdx.x >> 3);
     return x < W && y < H;
 }
+
+__host__ inline uint32_t tile_count(uint32_t W, uint32_t H) { return ((W + 7) / 8) * ((H + kTileH - 1) / kTileH); }
 
 struct Surf {
     V3 pos, nrm, col;
@@ -124,7 +137,7 @@ __device__ __forceinline__ float length3(V3 a) { return sqrtf(dot(a, a)); }
 template <bool COUNT>
 __global__ void __launch_bounds__(kProgBlock) k_ao_primary(ProgArgs A)
 {
-    __shared__ uint2 stack[kLdsStack * kProgBlock];
+    extern __shared__ uint2 stack[];
     uint32_t x, y;
     const bool active = tile_pixel(A.W, A.H, x, y);
     TravCounts tc{0, 0, 0, false};
@@ -189,9 +202,9 @@ __global__ void __launch_bounds__(kProgBlock) k_ao_primary(ProgArgs A)
 // one lane per (queue entry, AO ray); n_ao is a power of two <= 64, so the
 // rays of one pixel are adjacent lanes of one wave
 template <bool COUNT>
-__global__ void __launch_bounds__(kProgBlock) k_ao_rays(ProgArgs A)
+__global__ void __launch_bounds__(kProgBlock) SOLHIP_AO_RAYS_OCCUPANCY k_ao_rays(ProgArgs A)
 {
-    __shared__ uint2 stack[kLdsStack * kProgBlock];
+    extern __shared__ uint2 stack[];
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
     const uint64_t t = (uint64_t)lb * kProgBlock + threadIdx.x;
     const uint32_t count = *A.qcount;
@@ -232,6 +245,9 @@ __global__ void __launch_bounds__(kProgBlock) k_ao_rays(ProgArgs A)
     count_rays<COUNT>(A.counters + SOLHIP_KERNEL_AO_RAYS, active ? 1u : 0u, tc);
 }
 
+// dynamic LDS: the traversal stack, sv.stack_depth 8-byte entries per lane
+static size_t stack_bytes(const ProgArgs &a) { return (size_t)a.sv.stack_depth * sizeof(uint2) * kProgBlock; }
+
 static int launch_status(const char *what)
 {
     hipError_t e = hipGetLastError();
@@ -246,15 +262,15 @@ static int launch_status(const char *what)
     do {                                                                                      \
         if (timer) (timer)->begin(s, id);                                                     \
         if ((a).count)                                                                        \
-            hipLaunchKernelGGL(kernel<true>, dim3(grid), dim3(kProgBlock), 0, s, a);          \
+            hipLaunchKernelGGL(kernel<true>, dim3(grid), dim3(kProgBlock), stack_bytes(a), s, a); \
         else                                                                                  \
-            hipLaunchKernelGGL(kernel<false>, dim3(grid), dim3(kProgBlock), 0, s, a);         \
+            hipLaunchKernelGGL(kernel<false>, dim3(grid), dim3(kProgBlock), stack_bytes(a), s, a); \
         if (timer) (timer)->end(s, id);                                                       \
     } while (0)
 
 int launch_ao(hipStream_t s, const ProgArgs &a, Timer *timer)
 {
-    const uint32_t tiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
+    const uint32_t tiles = tile_count(a.W, a.H);
     SOLHIP_LAUNCH(k_ao_primary, SOLHIP_KERNEL_AO_PRIMARY, tiles, a, s, timer);
     const uint64_t lanes = (uint64_t)a.W * a.H * a.n_ao;
     const uint32_t blocks = (uint32_t)((lanes + kProgBlock - 1) / kProgBlock);
@@ -268,7 +284,7 @@ int launch_ao(hipStream_t s, const ProgArgs &a, Timer *timer)
 template <bool COUNT>
 __global__ void __launch_bounds__(kProgBlock) k_ao_glsl(ProgArgs A)
 {
-    __shared__ uint2 stack[kLdsStack * kProgBlock];
+    extern __shared__ uint2 stack[];
     uint32_t x, y;
     const bool active = tile_pixel(A.W, A.H, x, y);
     TravCounts tc{0, 0, 0, false};
@@ -332,7 +348,7 @@ __global__ void __launch_bounds__(kProgBlock) k_ao_glsl(ProgArgs A)
 
 int launch_ao_glsl(hipStream_t s, const ProgArgs &a, Timer *timer)
 {
-    const uint32_t tiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
+    const uint32_t tiles = tile_count(a.W, a.H);
     SOLHIP_LAUNCH(k_ao_glsl, SOLHIP_KERNEL_AO_GLSL, tiles, a, s, timer);
     return launch_status("AO-glsl");
 }
@@ -359,7 +375,7 @@ __device__ __forceinline__ V3 pt_miss(uint32_t enable_sky, V3 d)
 template <bool COUNT>
 __global__ void __launch_bounds__(kProgBlock) k_pathtrace_glsl(ProgArgs A)
 {
-    __shared__ uint2 stack[kLdsStack * kProgBlock];
+    extern __shared__ uint2 stack[];
     uint32_t x, y;
     const bool active = tile_pixel(A.W, A.H, x, y);
     TravCounts tc{0, 0, 0, false};
@@ -459,7 +475,7 @@ __global__ void __launch_bounds__(kProgBlock) k_pathtrace_glsl(ProgArgs A)
 
 int launch_pathtrace_glsl(hipStream_t s, const ProgArgs &a, Timer *timer)
 {
-    const uint32_t tiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
+    const uint32_t tiles = tile_count(a.W, a.H);
     SOLHIP_LAUNCH(k_pathtrace_glsl, SOLHIP_KERNEL_PATHTRACE_GLSL, tiles, a, s, timer);
     return launch_status("pathtrace");
 }
@@ -470,7 +486,7 @@ int launch_pathtrace_glsl(hipStream_t s, const ProgArgs &a, Timer *timer)
 template <bool COUNT>
 __global__ void __launch_bounds__(kProgBlock) k_debug(ProgArgs A)
 {
-    __shared__ uint2 stack[kLdsStack * kProgBlock];
+    extern __shared__ uint2 stack[];
     uint32_t x, y;
     const bool active = tile_pixel(A.W, A.H, x, y);
     TravCounts tc{0, 0, 0, false};
@@ -490,7 +506,7 @@ __global__ void __launch_bounds__(kProgBlock) k_debug(ProgArgs A)
 
 int launch_debug(hipStream_t s, const ProgArgs &a, Timer *timer)
 {
-    const uint32_t tiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
+    const uint32_t tiles = tile_count(a.W, a.H);
     SOLHIP_LAUNCH(k_debug, SOLHIP_KERNEL_DEBUG, tiles, a, s, timer);
     return launch_status("debug");
 }

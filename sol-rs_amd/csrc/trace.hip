@@ -38,7 +38,7 @@ template <bool ANY, bool COUNT>
 __global__ void __launch_bounds__(kTraceBlock) k_trace_rays(SceneView sv, const solhip_ray *rays, solhip_hit *hits,
                                                             uint32_t n, Counters *counters)
 {
-    __shared__ uint2 stack[kLdsStack * kTraceBlock];
+    extern __shared__ uint2 stack[];
     const uint32_t lb = xcd_logical_block(blockIdx.x, gridDim.x);
     const uint32_t i = lb * kTraceBlock + threadIdx.x;
     const bool active = i < n;
@@ -71,15 +71,16 @@ int launch_trace_rays(hipStream_t s, const SceneView &sv, const solhip_ray *rays
     if (n == 0) return 0;
     if (timer) timer->begin(s, SOLHIP_KERNEL_TRACE_RAYS);
     const dim3 grid((n + kTraceBlock - 1) / kTraceBlock), block(kTraceBlock);
+    const size_t lds = (size_t)sv.stack_depth * sizeof(uint2) * kTraceBlock;
     const bool any = flags & SOLHIP_TRACE_TERMINATE_ON_FIRST_HIT, count = flags & SOLHIP_TRACE_COUNT;
     if (any && count)
-        hipLaunchKernelGGL((k_trace_rays<true, true>), grid, block, 0, s, sv, rays, hits, n, counters);
+        hipLaunchKernelGGL((k_trace_rays<true, true>), grid, block, lds, s, sv, rays, hits, n, counters);
     else if (any)
-        hipLaunchKernelGGL((k_trace_rays<true, false>), grid, block, 0, s, sv, rays, hits, n, counters);
+        hipLaunchKernelGGL((k_trace_rays<true, false>), grid, block, lds, s, sv, rays, hits, n, counters);
     else if (count)
-        hipLaunchKernelGGL((k_trace_rays<false, true>), grid, block, 0, s, sv, rays, hits, n, counters);
+        hipLaunchKernelGGL((k_trace_rays<false, true>), grid, block, lds, s, sv, rays, hits, n, counters);
     else
-        hipLaunchKernelGGL((k_trace_rays<false, false>), grid, block, 0, s, sv, rays, hits, n, counters);
+        hipLaunchKernelGGL((k_trace_rays<false, false>), grid, block, lds, s, sv, rays, hits, n, counters);
     if (timer) timer->end(s, SOLHIP_KERNEL_TRACE_RAYS);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {

@@ -14,9 +14,10 @@
 // brute-force loop over all triangles with the same triangle test.
 //
 // Stack: node groups (child_base, remaining hit mask in visit order, imask)
-// of 8 bytes; the first kLdsStack entries live in LDS (one column per
-// lane), the next kPrivStack in a private array; deeper pushes are counted
-// as overflow (never expected: one entry per tree level).
+// of 8 bytes in LDS, one column per lane.  Each tree level pushes at most one
+// group, so the depth a launch needs is at most the TLAS depth plus the
+// deepest BLAS depth (SceneView::stack_depth, known at build time); a push
+// beyond it is counted as overflow and never happens on a consistent tree.
 #pragma once
 
 #include "common.h"
@@ -24,8 +25,8 @@
 
 namespace solhip {
 
-constexpr int kLdsStack = 12;
-constexpr int kPrivStack = 52;
+// largest stack (entries per lane) a launch may ask for
+constexpr int kMaxStack = 64;
 
 struct Hit {
     float t, u, v;
@@ -151,12 +152,35 @@ SOLHIP_HD V3 xform_dir(const float *m, V3 p)
 
 SOLHIP_HD float byte_f(uint32_t w, int b) { return (float)((w >> (8 * b)) & 0xFFu); }
 
-// Intersect the 8 child boxes of a node.  Returns internal-child hits in
-// visit order (bit i <-> slot i ^ oct) and the mask of leaf primitives
-// (relative to prim_base) of the leaf children hit.
-SOLHIP_HD void intersect_node(const uint4 w0, const uint4 w1, const uint4 w2, const uint4 w3,
-                                               const uint4 w4, V3 o, V3 inv, uint32_t oct, float tlo, float thi,
-                                               uint32_t &hits_int, uint32_t &prim_mask)
+// loads through the global address space (the pointers come from memory, so
+// the compiler cannot infer it and would otherwise emit flat loads)
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const __attribute__((address_space(1))) uint4 *gptr4;
+typedef const __attribute__((address_space(1))) uint2 *gptr2;
+typedef const __attribute__((address_space(1))) uint32_t *gptr1;
+#define SOLHIP_G4(p) ((gptr4)(p))
+#define SOLHIP_G2(p) ((gptr2)(p))
+#define SOLHIP_G1(p) ((gptr1)(p))
+#else
+#define SOLHIP_G4(p) ((const uint4 *)(p))
+#define SOLHIP_G2(p) ((const uint2 *)(p))
+#define SOLHIP_G1(p) ((const uint32_t *)(p))
+#endif
+
+// reverse the 8-bit slot-order mask into visit order: bit i <-> slot i ^ oct
+SOLHIP_HD uint32_t to_visit_order(uint32_t m, uint32_t oct)
+{
+    if (oct & 1u) m = ((m & 0x55u) << 1) | ((m & 0xAAu) >> 1);
+    if (oct & 2u) m = ((m & 0x33u) << 2) | ((m & 0xCCu) >> 2);
+    if (oct & 4u) m = ((m & 0x0Fu) << 4) | ((m & 0xF0u) >> 4);
+    return m;
+}
+
+// Intersect the 8 child boxes of a node (branch-free).  Returns
+// internal-child hits in visit order (bit i <-> slot i ^ oct) and the mask of
+// leaf primitives (relative to prim_base) of the leaf children hit.
+SOLHIP_HD void intersect_node(const uint4 w0, const uint4 w1, const uint4 w2, const uint4 w3, const uint4 w4, V3 o,
+                              V3 inv, uint32_t oct, float tlo, float thi, uint32_t &hits_int, uint32_t &prim_mask)
 {
     const float px = as_f(w0.x), py = as_f(w0.y), pz = as_f(w0.z);
     const uint32_t e = w0.w;
@@ -174,7 +198,7 @@ SOLHIP_HD void intersect_node(const uint4 w0, const uint4 w1, const uint4 w2, co
     const uint32_t ffy0 = fy ? w2.z : w4.x, ffy1 = fy ? w2.w : w4.y;
     const uint32_t nz0 = fz ? w4.z : w3.x, nz1 = fz ? w4.w : w3.y;
     const uint32_t ffz0 = fz ? w3.x : w4.z, ffz1 = fz ? w3.y : w4.w;
-    uint32_t hi_bits = 0, pm = 0;
+    uint32_t hm = 0, pm = 0;
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
         const int b = s & 3;
@@ -188,24 +212,25 @@ SOLHIP_HD void intersect_node(const uint4 w0, const uint4 w1, const uint4 w2, co
         const float tfz = fmaf(byte_f(afz, b), sz, bz);
         const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tlo));
         const float tf = fminf(fminf(tfx, tfy), fminf(tfz, thi));
+        const bool hit = tn <= tf;
+        // leaf slot: (count << 5) | offset; internal and empty slots: 0
         const uint32_t meta = ((s < 4 ? w1.z : w1.w) >> (8 * b)) & 0xFFu;
-        if (tn <= tf) {
-            if ((imask >> s) & 1u) {
-                hi_bits |= 1u << (s ^ oct);
-            } else if (meta) {
-                pm |= ((1u << (meta >> 5)) - 1u) << (meta & 31u);
-            }
-        }
+        const uint32_t lm = ((1u << (meta >> 5)) - 1u) << (meta & 31u);
+        hm |= hit ? (1u << s) : 0u;
+        pm |= hit ? lm : 0u;
     }
-    hits_int = hi_bits;
+    hits_int = to_visit_order(hm & imask, oct);
     prim_mask = pm;
 }
 
 // Two-level traversal.  lds: this lane's LDS stack column base, stride =
-// number of lanes sharing the array.
+// number of lanes sharing the array, sv.stack_depth entries per lane.  Stack
+// entries are 8-byte node groups (child_base, remaining hits in visit order |
+// imask << 8).  A BLAS is traversed by a nested loop that uses the stack
+// above the TLAS entries.
 template <bool ANY, bool COUNT>
-SOLHIP_HD Hit trace_ray(const SceneView sv, V3 wo, V3 wd, float tmin, float tmax, uint32_t cull,
-                                         uint2 *lds, int stride, TravCounts &cnt)
+SOLHIP_HD Hit trace_ray(const SceneView sv, V3 wo, V3 wd, float tmin, float tmax, uint32_t cull, uint2 *lds,
+                        int stride, TravCounts &cnt)
 {
     Hit best;
     best.t = tmax;
@@ -214,131 +239,130 @@ SOLHIP_HD Hit trace_ray(const SceneView sv, V3 wo, V3 wd, float tmin, float tmax
     best.inst = SOLHIP_MISS;
     best.prim = SOLHIP_MISS;
     if (sv.tnodes != nullptr) {
-        uint2 priv[kPrivStack];
-        const V3 winv = safe_inv(wd);
-        const uint32_t woct = octant(wd);
-        V3 o = wo, inv = winv;
-        uint32_t oct = woct;
-        Wt wt;
-        const WideNode *nodes = sv.tnodes;
-        const TriRec *tris = nullptr;
-        bool in_blas = false;
-        uint32_t cur_inst = 0;
-        uint32_t ng_base = 0, ng_hits = 1u << woct, ng_imask = 1u;
-        uint32_t tg_base = 0, tg_mask = 0;
+        const int depth = (int)sv.stack_depth;
         int sp = 0;
-        const float tlo = tmin - fabsf(tmin) * 0x1p-20f;
         auto push = [&](uint32_t x, uint32_t y) {
-            if (sp < kLdsStack) {
+            if (sp < depth) {
                 lds[sp * stride] = make_uint2(x, y);
-            } else if (sp < kLdsStack + kPrivStack) {
-                priv[sp - kLdsStack] = make_uint2(x, y);
+                ++sp;
             } else {
                 cnt.overflow = true;
-                return;
             }
-            ++sp;
         };
-        while (true) {
-            if (ng_hits) {
-                const uint32_t i = __builtin_ctz(ng_hits);
-                ng_hits &= ng_hits - 1u;
-                const uint32_t slot = i ^ oct;
-                const uint32_t child = ng_base + __builtin_popcount(ng_imask & ((1u << slot) - 1u));
-                if (ng_hits) push(ng_base, ng_hits | (ng_imask << 8));
-                const uint4 *np = reinterpret_cast<const uint4 *>(nodes + child);
-                const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
-                if (COUNT) cnt.nodes++;
-                const float thi = best.t + fabsf(best.t) * 0x1p-20f;
-                uint32_t hits_int, prim_mask;
-                intersect_node(w0, w1, w2, w3, w4, o, inv, oct, tlo, thi, hits_int, prim_mask);
-                ng_base = w1.x;
-                ng_imask = w0.w >> 24;
-                ng_hits = hits_int;
-                tg_base = w1.y;
-                tg_mask = prim_mask;
-            } else if (!tg_mask) {
-                if (sp == 0) break;
-                --sp;
-                const uint2 e = sp < kLdsStack ? lds[sp * stride] : priv[sp - kLdsStack];
-                if (e.y >> 16) {
-                    // instance marker: back to the TLAS with the remaining instances
-                    in_blas = false;
-                    nodes = sv.tnodes;
-                    o = wo;
-                    inv = winv;
-                    oct = woct;
-                    tg_base = e.x;
-                    tg_mask = e.y & 0xFFu;
-                    ng_hits = 0;
-                } else {
-                    ng_base = e.x;
-                    ng_hits = e.y & 0xFFu;
-                    ng_imask = (e.y >> 8) & 0xFFu;
-                }
-                continue;
-            }
-            if (in_blas) {
-                while (tg_mask) {
+        auto pop = [&]() -> uint2 {
+            --sp;
+            return lds[sp * stride];
+        };
+        const float tlo = tmin - fabsf(tmin) * 0x1p-20f;
+
+        // one BLAS in object space; true when ANY found a hit
+        auto trace_blas = [&](const WideNode *nodes, const TriRec *tris, V3 o, V3 d, uint32_t inst) -> bool {
+            const V3 inv = safe_inv(d);
+            const uint32_t oct = octant(d);
+            const Wt wt = wt_setup(o, d);
+            const int base = sp;
+            uint32_t ng_base = 0, ng_hits = 1u << oct, ng_imask = 1u;
+            uint32_t tg_base = 0, tg_mask = 0;
+            while (true) {
+                if (tg_mask) {
+                    // one triangle per iteration
                     const uint32_t k = __builtin_ctz(tg_mask);
                     tg_mask &= tg_mask - 1u;
                     const uint4 *tp = reinterpret_cast<const uint4 *>(tris + tg_base + k);
-                    const uint4 a = tp[0], b = tp[1];
-                    const uint2 c = *reinterpret_cast<const uint2 *>(tp + 2);
-                    const float vv[9] = {as_f(a.x), as_f(a.y), as_f(a.z),
-                                         as_f(a.w), as_f(b.x), as_f(b.y),
-                                         as_f(b.z), as_f(b.w), as_f(c.x)};
+                    const uint4 a = SOLHIP_G4(tp)[0], b = SOLHIP_G4(tp)[1];
+                    const uint2 c = SOLHIP_G2(tp + 2)[0];
+                    const float vv[9] = {as_f(a.x), as_f(a.y), as_f(a.z), as_f(a.w), as_f(b.x),
+                                         as_f(b.y), as_f(b.z), as_f(b.w), as_f(c.x)};
                     if (COUNT) cnt.tris++;
                     float t, u, v;
-                    if (wt_hit(wt, tmin, best.t, vv, t, u, v)) {
-                        const uint32_t prim = c.y;
-                        if (better(t, cur_inst, prim, best)) {
-                            best.t = t;
-                            best.u = u;
-                            best.v = v;
-                            best.inst = cur_inst;
-                            best.prim = prim;
-                            if (ANY) goto done;
+                    if (wt_hit(wt, tmin, best.t, vv, t, u, v) && better(t, inst, c.y, best)) {
+                        best.t = t;
+                        best.u = u;
+                        best.v = v;
+                        best.inst = inst;
+                        best.prim = c.y;
+                        if (ANY) {
+                            sp = base;
+                            return true;
                         }
                     }
+                } else if (ng_hits) {
+                    const uint32_t i = __builtin_ctz(ng_hits);
+                    ng_hits &= ng_hits - 1u;
+                    const uint32_t slot = i ^ oct;
+                    const uint32_t child = ng_base + __builtin_popcount(ng_imask & ((1u << slot) - 1u));
+                    if (ng_hits) push(ng_base, ng_hits | (ng_imask << 8));
+                    const uint4 *np = reinterpret_cast<const uint4 *>(nodes + child);
+                    const uint4 w0 = SOLHIP_G4(np)[0], w1 = SOLHIP_G4(np)[1], w2 = SOLHIP_G4(np)[2],
+                                w3 = SOLHIP_G4(np)[3], w4 = SOLHIP_G4(np)[4];
+                    if (COUNT) cnt.nodes++;
+                    const float thi = best.t + fabsf(best.t) * 0x1p-20f;
+                    intersect_node(w0, w1, w2, w3, w4, o, inv, oct, tlo, thi, ng_hits, tg_mask);
+                    ng_base = w1.x;
+                    ng_imask = w0.w >> 24;
+                    tg_base = w1.y;
+                } else if (sp > base) {
+                    const uint2 e = pop();
+                    ng_base = e.x;
+                    ng_hits = e.y & 0xFFu;
+                    ng_imask = (e.y >> 8) & 0xFFu;
+                } else {
+                    return false;
                 }
-            } else if (tg_mask) {
+            }
+        };
+
+        const V3 winv = safe_inv(wd);
+        const uint32_t woct = octant(wd);
+        uint32_t ng_base = 0, ng_hits = 1u << woct, ng_imask = 1u;
+        uint32_t tg_base = 0, tg_mask = 0;
+        while (true) {
+            if (tg_mask) {
+                // instance leaf: transform the ray and trace its BLAS
                 const uint32_t k = __builtin_ctz(tg_mask);
                 tg_mask &= tg_mask - 1u;
                 const uint32_t li = tg_base + k;
-                const uint32_t meta = sv.inst_meta[li];
+                const uint32_t meta = SOLHIP_G1(sv.inst_meta + li)[0];
                 if (((meta >> 24) & cull) == 0u) continue;
-                if (ng_hits) {
-                    push(ng_base, ng_hits | (ng_imask << 8));
-                    ng_hits = 0;
-                }
-                push(tg_base, tg_mask | (1u << 16));
-                tg_mask = 0;
                 const uint4 *ip = reinterpret_cast<const uint4 *>(sv.insts + li);
-                const uint4 i0 = ip[0], i1 = ip[1], i2 = ip[2], i3 = ip[3];
+                const uint4 i0 = SOLHIP_G4(ip)[0], i1 = SOLHIP_G4(ip)[1], i2 = SOLHIP_G4(ip)[2],
+                            i3 = SOLHIP_G4(ip)[3];
                 if (COUNT) cnt.insts++;
-                const WideNode *bn = reinterpret_cast<const WideNode *>(((uint64_t)i3.y << 32) | i3.x);
-                if (bn == nullptr) continue;
-                const float m[12] = {as_f(i0.x), as_f(i0.y), as_f(i0.z),
-                                     as_f(i0.w), as_f(i1.x), as_f(i1.y),
-                                     as_f(i1.z), as_f(i1.w), as_f(i2.x),
-                                     as_f(i2.y), as_f(i2.z), as_f(i2.w)};
-                o = xform_point(m, wo);
+                const uint64_t bn = ((uint64_t)i3.y << 32) | i3.x;
+                if (bn == 0) continue;
+                const float m[12] = {as_f(i0.x), as_f(i0.y), as_f(i0.z), as_f(i0.w), as_f(i1.x), as_f(i1.y),
+                                     as_f(i1.z), as_f(i1.w), as_f(i2.x), as_f(i2.y), as_f(i2.z), as_f(i2.w)};
+                const V3 o = xform_point(m, wo);
                 const V3 d = xform_dir(m, wd);
-                inv = safe_inv(d);
-                oct = octant(d);
-                wt = wt_setup(o, d);
-                nodes = bn;
-                tris = reinterpret_cast<const TriRec *>(((uint64_t)i3.w << 32) | i3.z);
-                in_blas = true;
-                cur_inst = meta & 0xFFFFFFu;
-                ng_base = 0;
-                ng_imask = 1u;
-                ng_hits = 1u << oct;
+                const uint64_t tr = ((uint64_t)i3.w << 32) | i3.z;
+                if (trace_blas(reinterpret_cast<const WideNode *>(bn), reinterpret_cast<const TriRec *>(tr), o, d,
+                               meta & 0xFFFFFFu))
+                    break;
+            } else if (ng_hits) {
+                const uint32_t i = __builtin_ctz(ng_hits);
+                ng_hits &= ng_hits - 1u;
+                const uint32_t slot = i ^ woct;
+                const uint32_t child = ng_base + __builtin_popcount(ng_imask & ((1u << slot) - 1u));
+                if (ng_hits) push(ng_base, ng_hits | (ng_imask << 8));
+                const uint4 *np = reinterpret_cast<const uint4 *>(sv.tnodes + child);
+                const uint4 w0 = SOLHIP_G4(np)[0], w1 = SOLHIP_G4(np)[1], w2 = SOLHIP_G4(np)[2],
+                            w3 = SOLHIP_G4(np)[3], w4 = SOLHIP_G4(np)[4];
+                if (COUNT) cnt.nodes++;
+                const float thi = best.t + fabsf(best.t) * 0x1p-20f;
+                intersect_node(w0, w1, w2, w3, w4, wo, winv, woct, tlo, thi, ng_hits, tg_mask);
+                ng_base = w1.x;
+                ng_imask = w0.w >> 24;
+                tg_base = w1.y;
+            } else if (sp > 0) {
+                const uint2 e = pop();
+                ng_base = e.x;
+                ng_hits = e.y & 0xFFu;
+                ng_imask = (e.y >> 8) & 0xFFu;
+            } else {
+                break;
             }
         }
     }
-done:
     if (best.inst == SOLHIP_MISS) {
         best.t = -1.0f;
         best.u = 0.0f;

@@ -11,7 +11,7 @@ import numpy as np
 from .records import HIT, INSTANCE_DESC, MATERIAL, RAY, SCENE_INSTANCE, UNIFORMS
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "..", "lib", "libsolhip.so"))
+LIB_PATH = os.environ.get("SOLHIP_LIB") or os.path.normpath(os.path.join(_HERE, "..", "..", "lib", "libsolhip.so"))
 HEADER_PATH = os.path.normpath(os.path.join(_HERE, "..", "..", "..", "include", "solhip.h"))
 
 PROG_AO_GLSL, PROG_PATHTRACE_GLSL, PROG_DEBUG, PROG_AO = 1, 2, 3, 4

@@ -217,7 +217,10 @@ void hc_trace(void *h, const float *rays, uint32_t n, uint32_t any, void *hits, 
     sv.tnodes = s->insts.empty() ? nullptr : s->tlas.nodes.data();
     sv.insts = s->insts.data();
     sv.inst_meta = s->meta.data();
-    uint2 lds[kLdsStack];
+    uint32_t max_blas_depth = 0;
+    for (const auto &b : s->blas) max_blas_depth = std::max(max_blas_depth, b.bvh.depth);
+    sv.stack_depth = std::min<uint32_t>(kMaxStack, s->tlas.depth + max_blas_depth + 1);
+    uint2 lds[kMaxStack];
     for (uint32_t i = 0; i < n; ++i) {
         const float *r = rays + 8 * (size_t)i;
         TravCounts tc{0, 0, 0, false};
