@@ -144,7 +144,10 @@ enum {
     SOLHIP_PROG_AO_GLSL = 1,        /* ao.rgen/rchit/rmiss */
     SOLHIP_PROG_PATHTRACE_GLSL = 2, /* pathtrace.rgen/rchit/rmiss */
     SOLHIP_PROG_DEBUG = 3,          /* debug.rgen/rchit/rmiss */
-    SOLHIP_PROG_AO = 4              /* plain ray-traced AO (primary + n_ao rays) */
+    SOLHIP_PROG_AO = 4,             /* plain ray-traced AO (primary + n_ao rays) */
+    SOLHIP_PROG_PATHTRACE = 5       /* wavefront path tracer: pathtrace.* estimator,
+                                       optional next-event estimation and
+                                       Russian roulette */
 };
 
 /* specialization + run configuration; solhip_spec_default gives the GLSL
@@ -156,7 +159,16 @@ typedef struct {
     uint32_t n_ao;            /* SOLHIP_PROG_AO: rays per primary hit (8) */
     float ao_tmin, ao_tmax;   /* SOLHIP_PROG_AO ray range */
     uint32_t ao_any_hit;      /* SOLHIP_PROG_AO: occlusion may stop at any hit */
-    uint32_t reserved[9];
+    /* SOLHIP_PROG_PATHTRACE.  With rr_start = 0 and nee = 0 the image is
+     * bit-identical to SOLHIP_PROG_PATHTRACE_GLSL's.  enable_sky = 2 gives the
+     * sky gradient of pathtrace.rmiss without the sun disc. */
+    uint32_t rr_start;        /* Russian roulette from this scattering vertex on; 0 = off */
+    uint32_t nee;             /* next-event estimation to emissive triangles at diffuse vertices */
+    uint32_t wf_blocks_per_cu;/* persistent grid of the extend / shadow kernels; 0 = default */
+    uint32_t wf_block;        /* threads per block of the wavefront kernels (64..256); 0 = default */
+    uint32_t wf_graph;        /* 1: replay the frame's passes as a captured hipGraph (recaptured
+                                 when buffers, scene or launch shape change) */
+    uint32_t reserved[4];
 } solhip_spec;
 
 /* per-dispatch images: W*H pixels, row-major, no padding */
@@ -244,7 +256,13 @@ enum {
     SOLHIP_KERNEL_AO_GLSL = 3,
     SOLHIP_KERNEL_PATHTRACE_GLSL = 4,
     SOLHIP_KERNEL_DEBUG = 5,
-    SOLHIP_KERNEL_COUNT = 6
+    SOLHIP_KERNEL_PT_GENERATE = 6,  /* wavefront path tracer: camera rays, per-pixel sums */
+    SOLHIP_KERNEL_PT_EXTEND = 7,    /* closest hits of the extension queue */
+    SOLHIP_KERNEL_PT_SHADE = 8,     /* materials, emission, scattering, queue compaction */
+    SOLHIP_KERNEL_PT_SHADOW = 9,    /* any-hit shadow rays of next-event estimation */
+    SOLHIP_KERNEL_PT_FINALIZE = 10, /* sample mean, accumulation, RGBA8 */
+    SOLHIP_KERNEL_PT_LIGHTS = 11,   /* emitter table (next-event estimation) */
+    SOLHIP_KERNEL_COUNT = 12
 };
 #define SOLHIP_PROFILE_TIME   0x1u  /* hipEvents around every launch, on the launch stream */
 #define SOLHIP_PROFILE_COUNT  0x2u  /* fetch-counting kernel variants */
@@ -255,6 +273,21 @@ typedef struct {
     solhip_counters counters;  /* rays / fetches of this kernel */
 } solhip_kernel_stats;
 int32_t solhip_kernel_stats_get(solhip_ctx ctx, uint32_t kernel, solhip_kernel_stats *out, uint32_t reset);
+
+/* ---- launch-shape autotuner for SOLHIP_PROG_PATHTRACE (no counterpart in
+ * the reference, whose dispatch shape is the driver's) --------------------
+ * Times `frames` frames of every candidate launch shape (wf_block in
+ * {64, 128, 256} x three persistent grid sizes x wf_graph in {0, 1}) for the
+ * estimator settings in *spec on this scene and image size, and returns *spec
+ * with the fastest shape in *best (and its frame time).  Results persist in
+ * cache_path (text, one line per workload: device, CUs, size, estimator
+ * settings, instance and triangle counts); a cached workload is answered
+ * without timing.  The timed frames render into images->accum / out_rgba8.
+ * cache_path may be NULL (no persistence). */
+int32_t solhip_pathtrace_autotune(solhip_ctx ctx, solhip_stream s, solhip_tlas t, const solhip_scene_tables *tables,
+                                  const solhip_scene_uniforms *uniforms, const solhip_images *images, uint32_t W,
+                                  uint32_t H, const solhip_spec *spec, uint32_t frames, const char *cache_path,
+                                  solhip_spec *best, float *best_ms);
 
 #ifdef __cplusplus
 }

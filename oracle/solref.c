@@ -138,11 +138,10 @@ static void sr_sincos(float x, float *s, float *c)
 float sr_sinf(float x) { float s, c; sr_sincos(x, &s, &c); return s; }
 float sr_cosf(float x) { float s, c; sr_sincos(x, &s, &c); return c; }
 
-/* assets/glsl/sampling.glsl:52-64 */
-float sr_fresnel_dielectric(const float i[3], const float m[3], float eta)
+/* assets/glsl/sampling.glsl:52-64, as a function of cos(theta_i) */
+static float fresnel_cos(float cosThetaI, float eta)
 {
     float result = 1.0f;
-    float cosThetaI = fabsf(dot3(i, m));
     float sinThetaOSquared = (eta * eta) * (1.0f - cosThetaI * cosThetaI);
     if (sinThetaOSquared <= 1.0f) {
         float cosThetaO = sqrtf(saturate(1.0f - sinThetaOSquared));
@@ -151,6 +150,12 @@ float sr_fresnel_dielectric(const float i[3], const float m[3], float eta)
         result = 0.5f * (Rs * Rs + Rp * Rp);
     }
     return result;
+}
+
+/* assets/glsl/sampling.glsl:52-64 */
+float sr_fresnel_dielectric(const float i[3], const float m[3], float eta)
+{
+    return fresnel_cos(fabsf(dot3(i, m)), eta);
 }
 
 /* assets/glsl/sampling.glsl:66-73 (Duff et al. 2017 basis) */
@@ -864,7 +869,7 @@ void sr_render_ao(const sr_scene *s, const sr_tables *tb, const sr_uniforms *un,
 /* ------------------------------------------------------------------------ */
 /* pathtrace.rgen / pathtrace.rchit / pathtrace.rmiss                       */
 /* ------------------------------------------------------------------------ */
-static void pt_miss(uint32_t enable_sky, const float d[3], float hv[3])
+static void pt_miss_sun(uint32_t enable_sky, int sun, const float d[3], float hv[3])
 {
     if (enable_sky) {                                      /* pathtrace.rmiss:10-16 */
         float wI[3];
@@ -876,7 +881,7 @@ static void pt_miss(uint32_t enable_sky, const float d[3], float hv[3])
         const float sd0[3] = {0.0f, 1.0f, -0.25f};
         float sd[3];
         normalize3(sd0, sd);
-        float isSun = dot3(wI, sd) > 0.99f ? 1.0f : 0.0f;
+        float isSun = (sun && dot3(wI, sd) > 0.99f) ? 1.0f : 0.0f;
         const float sun[3] = {120.0f, 100.0f, 50.0f};
         for (int a = 0; a < 3; ++a) {
             float sky = mixf(c0[a], c1[a], t);
@@ -886,6 +891,8 @@ static void pt_miss(uint32_t enable_sky, const float d[3], float hv[3])
         hv[0] = hv[1] = hv[2] = 0.0f;
     }
 }
+
+static void pt_miss(uint32_t enable_sky, const float d[3], float hv[3]) { pt_miss_sun(enable_sky, 1, d, hv); }
 
 static void pt_glsl_pixel(const sr_scene *s, const sr_tables *tb, const sr_uniforms *un,
                           const sr_pt_params *p, int32_t accum_start, float *accum,
@@ -998,6 +1005,230 @@ void sr_render_pathtrace_glsl(const sr_scene *s, const sr_tables *tb, const sr_u
             pt_glsl_pixel(s, tb, u, p, accum_start_frame, accum, out_rgba8, &local, x, (uint32_t)y, W, H);
         rays += local;
     }
+    if (ray_count) *ray_count = rays;
+}
+
+/* ------------------------------------------------------------------------ */
+/* wavefront path tracer's estimator (SOLHIP_PROG_PATHTRACE)                */
+/* ------------------------------------------------------------------------ */
+static inline int is_emitter(const sr_material *m)
+{
+    return m->emissive[0] >= 1.0f || m->emissive[1] >= 1.0f || m->emissive[2] >= 1.0f;   /* pathtrace.rchit:71 */
+}
+
+uint32_t sr_lights(const sr_scene *s, const sr_tables *tb, sr_light *out, uint32_t cap, float *total_area)
+{
+    uint32_t n = 0;
+    float run = 0.0f;
+    uint32_t ninst = s->ninst < tb->instance_count ? s->ninst : tb->instance_count;
+    for (uint32_t i = 0; tb->materials && i < ninst; ++i) {
+        const sr_material *m = tb->materials[i];
+        if (!is_emitter(m)) continue;
+        const sr_scene_instance *si = &tb->instances[i];
+        const uint32_t obj = si->id, nt = s->blas[s->inst[i].blas].ntri;
+        for (uint32_t t = 0; t < nt; ++t) {
+            float w[3][4];
+            for (int k = 0; k < 3; ++k) {
+                const sr_model_vertex *v = &tb->vertices[obj][fetch_index(tb, obj, 3 * t + k)];
+                const float p[4] = {v->pos[0], v->pos[1], v->pos[2], 1.0f};
+                mat4_mul_vec4(si->transform, p, w[k]);
+            }
+            const float e1[3] = {w[1][0] - w[0][0], w[1][1] - w[0][1], w[1][2] - w[0][2]};
+            const float e2[3] = {w[2][0] - w[0][0], w[2][1] - w[0][1], w[2][2] - w[0][2]};
+            const float c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                                e1[0] * e2[1] - e1[1] * e2[0]};
+            const float len = sqrtf(dot3(c, c));
+            run = run + 0.5f * len;
+            if (n < cap) {
+                sr_light *L = &out[n];
+                for (int a = 0; a < 3; ++a) {
+                    L->p0[a] = w[0][a];
+                    L->p1[a] = w[1][a];
+                    L->p2[a] = w[2][a];
+                    L->n[a] = len > 0.0f ? c[a] / len : 0.0f;
+                    L->e[a] = m->emissive[a];
+                }
+                L->cdf = run;
+            }
+            ++n;
+        }
+    }
+    if (total_area) *total_area = run;
+    return n;
+}
+
+/* first light whose inclusive area sum exceeds u (the last one if none) */
+static inline uint32_t pick_light(const sr_light *L, uint32_t n, float u)
+{
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (L[mid].cdf > u) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
+
+static void pt_cfg_pixel(const sr_scene *s, const sr_tables *tb, const sr_uniforms *un,
+                         const sr_pt_config *p, const sr_light *lights, uint32_t nl, float total,
+                         int32_t accum_start, float *accum, uint8_t *out, uint64_t *rays,
+                         uint32_t x, uint32_t y, uint32_t W, uint32_t H)
+{
+    const int maxBounces = (int)p->max_bounces, sampleCount = (int)p->samples;
+    const int sun = p->enable_sky != 2;
+    const int nee = p->nee && nl > 0 && total > 0.0f;
+    float pixelColor[3] = {0.0f, 0.0f, 0.0f};
+    uint32_t rng = sr_tea(x + y * W, un->frame[2]);
+    for (int i = 0; i < sampleCount; i++) {
+        float jx = sr_next_rand(&rng);
+        float jy = sr_next_rand(&rng);
+        float o[3], d[3];
+        sr_camera_ray(un, x, y, W, H, jx, jy, o, d);
+        const float rmin = fmaxf(1.0f, length3(o)) * 1e-3f, rmax = 10000.0f;
+        uint32_t depth = 0;
+        int count_emit = 1;
+        float T[3] = {1.0f, 1.0f, 1.0f}, L[3] = {0.0f, 0.0f, 0.0f};
+        for (;;) {
+            sr_ray ray;
+            sr_hit h;
+            mk_ray(&ray, o, rmin, d, rmax);
+            trace_one(s, &ray, &h, 1, 0xFFu, 0);
+            *rays += 1;
+            if (h.inst == SR_MISS) {
+                float sv[3];
+                pt_miss_sun(p->enable_sky, sun, d, sv);
+                for (int a = 0; a < 3; ++a) L[a] = L[a] + T[a] * sv[a];
+                break;
+            }
+            const sr_material *mat = tb->materials[h.inst];
+            if (is_emitter(mat)) {
+                if (count_emit)
+                    for (int a = 0; a < 3; ++a) L[a] = L[a] + T[a] * mat->emissive[a];
+                break;
+            }
+            surf_t sf;
+            fetch_surface(tb, &h, &sf);
+            float wI[3];
+            normalize3(d, wI);
+            float nwI[3] = {-wI[0], -wI[1], -wI[2]};
+            float sg = signf_(dot3(sf.nrm, nwI));
+            float nO[3] = {sf.nrm[0] * sg, sf.nrm[1] * sg, sf.nrm[2] * sg};
+            float alphaSquared = mat->roughness * mat->roughness;
+            float xi[2];
+            xi[0] = sr_next_rand(&rng);
+            xi[1] = sr_next_rand(&rng);
+            float rnd = sr_next_rand(&rng);
+            for (int a = 0; a < 3; ++a) o[a] = sf.pos[a] + 0.0001f * nO[a];
+            float nd[3], hv[3];
+            int diffuse = 0;
+            if (rnd < mat->metallic) {
+                float rf[3];
+                reflect3(d, nO, rf);
+                sr_sample_ggx(rf, xi, alphaSquared, nd);
+                for (int a = 0; a < 3; ++a) hv[a] = mat->base_color[a] * sf.col[a];
+            } else {
+                float m[3];
+                sr_sample_ggx(nO, xi, alphaSquared, m);
+                if (rnd < sr_fresnel_dielectric(nO, m, 1.0f / 1.5f)) {
+                    reflect3(d, m, nd);
+                    hv[0] = hv[1] = hv[2] = 1.0f;
+                } else {
+                    sr_sample_cosine(nO, xi, nd);
+                    for (int a = 0; a < 3; ++a) hv[a] = mat->base_color[a] * sf.col[a];
+                    diffuse = 1;
+                }
+            }
+            float Tb[3];   /* throughput into this vertex times its diffuse albedo */
+            for (int a = 0; a < 3; ++a) Tb[a] = T[a] * (mat->base_color[a] * sf.col[a]);
+            for (int a = 0; a < 3; ++a) T[a] = T[a] * hv[a];
+            d[0] = nd[0]; d[1] = nd[1]; d[2] = nd[2];
+            depth++;
+            if ((int)depth > maxBounces) break;
+            count_emit = 1;
+            if (nee) {
+                /* pathtrace.rchit:93-110 picks the diffuse lobe when
+                 * rand >= max(metallic, F(m)), with m drawn from the same Xi
+                 * as the cosine direction: m shares its azimuth and has
+                 * cos^2 theta_m = (1 - c^2) / (c^2 (a^2 - 1) + 1) for a
+                 * direction at cos theta = c.  The diffuse lobe's share of
+                 * direct light is therefore estimated with weight
+                 * 1 - max(metallic, F(theta_m)) at every vertex. */
+                const float ul = sr_next_rand(&rng) * total;
+                const float b1 = sr_next_rand(&rng), b2 = sr_next_rand(&rng);
+                const sr_light *Lt = &lights[pick_light(lights, nl, ul)];
+                const float su = sqrtf(b1), ba = 1.0f - su, bb = su * (1.0f - b2), bc = su * b2;
+                float dl[3];
+                for (int a = 0; a < 3; ++a) dl[a] = (Lt->p0[a] * ba + Lt->p1[a] * bb + Lt->p2[a] * bc) - o[a];
+                const float dist2 = dot3(dl, dl), dist = sqrtf(dist2);
+                float wl[3];
+                for (int a = 0; a < 3; ++a) wl[a] = dl[a] / dist;
+                const float cos_x = dot3(nO, wl), cos_l = fabsf(dot3(Lt->n, wl));
+                if (dist2 > 0.0f && cos_x > 0.0f && cos_l > 0.0f) {
+                    const float x0 = cos_x * cos_x;
+                    const float cm = sqrtf(saturate((1.0f - x0) / (x0 * (alphaSquared - 1.0f) + 1.0f)));
+                    const float pd = 1.0f - fmaxf(mat->metallic, fresnel_cos(cm, 1.0f / 1.5f));
+                    if (pd > 0.0f) {
+                        const float g = (pd * (cos_x * cos_l)) * total / (0x1.921fb6p+1f * dist2);
+                        sr_ray sray;
+                        sr_hit sh;
+                        mk_ray(&sray, o, rmin, wl, dist * 0.9999f);
+                        trace_one(s, &sray, &sh, 1, 0xFFu, 1);
+                        *rays += 1;
+                        if (sh.inst == SR_MISS)
+                            for (int a = 0; a < 3; ++a) L[a] = L[a] + (Tb[a] * Lt->e[a]) * g;
+                    }
+                }
+                /* emitters met by the diffuse lobe were sampled above */
+                if (diffuse) count_emit = 0;
+            }
+            if (p->rr_start && depth >= p->rr_start) {
+                const float q = fminf(fmaxf(T[0], fmaxf(T[1], T[2])), 0.95f);
+                if (sr_next_rand(&rng) >= q) break;
+                for (int a = 0; a < 3; ++a) T[a] = T[a] / q;
+            }
+        }
+        for (int a = 0; a < 3; ++a) pixelColor[a] = pixelColor[a] + L[a];
+    }
+    float inv = 1.0f / (float)sampleCount;
+    for (int a = 0; a < 3; ++a) pixelColor[a] = inv * pixelColor[a];
+    float alpha = 1.0f / (float)(un->frame[2] + 1u - (uint32_t)accum_start);
+    float *px = &accum[4 * ((size_t)y * W + x)];
+    float mixed[3];
+    int bad = 0;
+    for (int a = 0; a < 3; ++a) {
+        mixed[a] = mixf(px[a], pixelColor[a], alpha);
+        if (isnan(mixed[a]) || isinf(mixed[a])) bad = 1;
+    }
+    if (bad) { mixed[0] = px[0]; mixed[1] = px[1]; mixed[2] = px[2]; }
+    px[0] = mixed[0]; px[1] = mixed[1]; px[2] = mixed[2]; px[3] = 1.0f;
+    if (out) {
+        uint8_t *q = &out[4 * ((size_t)y * W + x)];
+        for (int a = 0; a < 3; ++a) q[a] = to_unorm8(powf(mixed[a], 1.0f / 2.2f));
+        q[3] = 255;
+    }
+}
+
+void sr_render_pathtrace(const sr_scene *s, const sr_tables *tb, const sr_uniforms *u,
+                         const sr_pt_config *p, int32_t accum_start_frame,
+                         float *accum, uint8_t *out_rgba8, uint64_t *ray_count,
+                         uint32_t W, uint32_t H, uint32_t y0, uint32_t y1, int nthreads)
+{
+    float total = 0.0f;
+    uint32_t nl = p->nee ? sr_lights(s, tb, NULL, 0, &total) : 0;
+    sr_light *lights = (sr_light *)malloc(sizeof(sr_light) * (nl ? nl : 1));
+    if (nl) sr_lights(s, tb, lights, nl, &total);
+    uint64_t rays = 0;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(+ : rays)
+#endif
+    for (int64_t y = y0; y < (int64_t)y1; ++y) {
+        uint64_t local = 0;
+        for (uint32_t x = 0; x < W; ++x)
+            pt_cfg_pixel(s, tb, u, p, lights, nl, total, accum_start_frame, accum, out_rgba8, &local, x,
+                         (uint32_t)y, W, H);
+        rays += local;
+    }
+    free(lights);
     if (ray_count) *ray_count = rays;
 }
 

@@ -166,6 +166,43 @@ void sr_render_pathtrace_glsl(const sr_scene *s, const sr_tables *tb, const sr_u
                               float *accum, uint8_t *out_rgba8, uint64_t *ray_count,
                               uint32_t W, uint32_t H, uint32_t y0, uint32_t y1, int nthreads);
 
+/* Wavefront path tracer (SOLHIP_PROG_PATHTRACE) restated as a per-pixel
+ * loop.  With nee = 0 and rr_start = 0 it is exactly the pathtrace.* program
+ * above (same RNG stream, same draws, same products).  Extensions, each
+ * consuming its draws after the three of pathtrace.rchit:93-94:
+ *  - next-event estimation of the diffuse lobe at every scattering vertex:
+ *    one emissive triangle picked by area (draw 1), a uniform point on it
+ *    (draws 2-3), an any-hit shadow ray, weighted by the probability
+ *    1 - max(metallic, F) that pathtrace.rchit takes the diffuse lobe for
+ *    that direction; when the diffuse lobe was sampled, the next vertex
+ *    ignores emitters it hits (they were sampled);
+ *  - Russian roulette from scattering vertex rr_start on: survival
+ *    q = min(max(T), 0.95) (draw 4), T /= q.
+ * A path dropped at max_bounces keeps what next-event estimation gave it.
+ * enable_sky: 0 black, 1 pathtrace.rmiss sky with sun, 2 its gradient only. */
+typedef struct {
+    uint32_t enable_sky;
+    uint32_t samples;
+    uint32_t max_bounces;
+    uint32_t rr_start;    /* 0 = no Russian roulette */
+    uint32_t nee;         /* 1 = next-event estimation */
+} sr_pt_config;
+void sr_render_pathtrace(const sr_scene *s, const sr_tables *tb, const sr_uniforms *u,
+                         const sr_pt_config *p, int32_t accum_start_frame,
+                         float *accum, uint8_t *out_rgba8, uint64_t *ray_count,
+                         uint32_t W, uint32_t H, uint32_t y0, uint32_t y1, int nthreads);
+
+/* light table of the estimator: every triangle of every instance whose
+ * material is an emitter (some emissive channel >= 1, pathtrace.rchit:71), in
+ * (instance, primitive) order, world-space vertices, unit geometric normal,
+ * emission and the running (inclusive) area sum.  Returns the count (writes
+ * at most cap records) and the total area. */
+typedef struct {
+    float p0[3], p1[3], p2[3], n[3], e[3];
+    float cdf;
+} sr_light;
+uint32_t sr_lights(const sr_scene *s, const sr_tables *tb, sr_light *out, uint32_t cap, float *total_area);
+
 /* debug.rgen/rchit/rmiss: barycentrics on hit, ray direction on miss */
 void sr_render_debug(const sr_scene *s, const sr_uniforms *u, float *out_rgba32f,
                      uint32_t W, uint32_t H, int nthreads);

@@ -37,6 +37,15 @@ class PTParams(C.Structure):
     _fields_ = [("enable_sky", C.c_uint32), ("samples", C.c_uint32), ("max_bounces", C.c_uint32)]
 
 
+class PTConfig(C.Structure):
+    """sr_pt_config: the wavefront path tracer's estimator options"""
+    _fields_ = [("enable_sky", C.c_uint32), ("samples", C.c_uint32), ("max_bounces", C.c_uint32),
+                ("rr_start", C.c_uint32), ("nee", C.c_uint32)]
+
+
+LIGHT_DTYPE = np.dtype([("p0", "<f4", 3), ("p1", "<f4", 3), ("p2", "<f4", 3), ("n", "<f4", 3), ("e", "<f4", 3),
+                        ("cdf", "<f4")])
+
 _lib = None
 
 
@@ -74,6 +83,9 @@ def lib():
         L.sr_render_ao.argtypes = [vp, vp, vp, vp, i32, vp, vp, vp, vp, u32, u32, u32, u32, C.c_int]
         L.sr_render_pathtrace_glsl.argtypes = [vp, vp, vp, vp, i32, vp, vp, vp, u32, u32, u32, u32, C.c_int]
         L.sr_render_debug.argtypes = [vp, vp, vp, u32, u32, C.c_int]
+        L.sr_render_pathtrace.argtypes = [vp, vp, vp, vp, i32, vp, vp, vp, u32, u32, u32, u32, C.c_int]
+        L.sr_lights.restype = u32
+        L.sr_lights.argtypes = [vp, vp, vp, u32, C.POINTER(f32)]
         L.sr_instance_inverse.argtypes = [vp, vp]
         L.sr_wall_seconds.restype = C.c_double
         _lib = L
@@ -255,6 +267,22 @@ class Scene:
                                        _p(out_rgba8) if out_rgba8 is not None else None,
                                        _p(cnt), W, H, y0, y1, nthreads)
         return int(cnt[0])
+
+    def render_pathtrace(self, uniforms, cfg, accum_start, accum, out_rgba8, W, H, y0=0, y1=None, nthreads=0):
+        """the wavefront path tracer's estimator (PTConfig) as a per-pixel loop"""
+        y1 = H if y1 is None else y1
+        cnt = np.zeros(1, np.uint64)
+        lib().sr_render_pathtrace(self.h, C.byref(self.tables), _p(uniforms), C.byref(cfg), accum_start, _p(accum),
+                                  _p(out_rgba8) if out_rgba8 is not None else None, _p(cnt), W, H, y0, y1, nthreads)
+        return int(cnt[0])
+
+    def lights(self):
+        """(light records, total area) of the estimator's emitter table"""
+        tot = C.c_float(0.0)
+        n = lib().sr_lights(self.h, C.byref(self.tables), None, 0, C.byref(tot))
+        out = np.zeros(max(n, 1), LIGHT_DTYPE)
+        lib().sr_lights(self.h, C.byref(self.tables), _p(out), n, C.byref(tot))
+        return out[:n], float(tot.value)
 
     def render_debug(self, uniforms, W, H, nthreads=0):
         out = np.zeros((H, W, 4), np.float32)

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -59,6 +60,7 @@ struct EventTimer : Timer {
 
 struct Context : Obj {
     int device = 0;
+    uint32_t ncu = 256;   // compute units (persistent grids)
     hipStream_t stream = nullptr;
     Counters *counters = nullptr;  // [SOLHIP_KERNEL_COUNT]
     uint32_t profile = 0;
@@ -101,12 +103,40 @@ struct Tlas : Obj {
     DevBuf inst_dev;   // solhip_instance[]
     DevBuf refs_dev;   // BlasRef[]
     std::vector<uint64_t> blas_handles;   // distinct BLAS referenced by the instances
+    uint32_t n_inst = 0;
+    DevBuf inst_tris_dev;                 // u32 triangle count of each instance's BLAS
 };
 
 struct Program : Obj {
     uint32_t kind_id = 0;
     solhip_spec spec{};
     DevBuf queue, qcount;
+    // SOLHIP_PROG_PATHTRACE: path state, queues, emitter table, frame constants
+    DevBuf st0, st1, psum, q0, q1, hits, sq, scontrib, counts, heads, lights, light_offsets, fc;
+    // captured frame (wf_graph): valid for the WfArgs it was captured with
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    WfArgs gkey{};
+    bool gvalid = false;
+    uint64_t graph_captures = 0;
+    void drop_graph()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+        gvalid = false;
+    }
+    void release_all()
+    {
+        drop_graph();
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
+        cap_stream = nullptr;
+        for (DevBuf *b : {&queue, &qcount, &st0, &st1, &psum, &q0, &q1, &hits, &sq, &scontrib, &counts, &heads,
+                          &lights, &light_offsets, &fc})
+            b->release();
+    }
 };
 
 std::mutex g_mu;
@@ -202,6 +232,7 @@ SceneView view_of(const Tlas *t)
 int tlas_build_into(Context *cx, hipStream_t s, Tlas *t, const solhip_instance *inst, uint32_t n)
 {
     std::vector<BlasRef> refs(n);
+    std::vector<uint32_t> tris(n ? n : 1, 0u);
     t->blas_handles.clear();
     for (uint32_t i = 0; i < n; ++i) {
         Blas *b = get<Blas>(inst[i].blas, Kind::Blas);
@@ -220,18 +251,129 @@ int tlas_build_into(Context *cx, hipStream_t s, Tlas *t, const solhip_instance *
         }
         refs[i].root_box = (const float *)b->bvh.root_box.ptr;
         refs[i].pad = 0;
+        tris[i] = b->bvh.prim_count;
     }
     const uint64_t nn = n ? n : 1;
-    if (t->inst_dev.reserve(nn * sizeof(solhip_instance)) || t->refs_dev.reserve(nn * sizeof(BlasRef)))
+    if (t->inst_dev.reserve(nn * sizeof(solhip_instance)) || t->refs_dev.reserve(nn * sizeof(BlasRef)) ||
+        t->inst_tris_dev.reserve(nn * sizeof(uint32_t)))
         return SOLHIP_ERROR_OUT_OF_MEMORY;
+    t->n_inst = n;
     if (n) {
         SOLHIP_HIP_CHECK(hipStreamSynchronize(s));
+        SOLHIP_HIP_CHECK(hipMemcpy(t->inst_tris_dev.ptr, tris.data(), (uint64_t)n * sizeof(uint32_t), hipMemcpyHostToDevice));
         SOLHIP_HIP_CHECK(hipMemcpy(t->inst_dev.ptr, inst, (uint64_t)n * sizeof(solhip_instance), hipMemcpyHostToDevice));
         SOLHIP_HIP_CHECK(hipMemcpy(t->refs_dev.ptr, refs.data(), (uint64_t)n * sizeof(BlasRef), hipMemcpyHostToDevice));
     }
     (void)cx;
     return build_tlas(s, (const solhip_instance *)t->inst_dev.ptr, (const BlasRef *)t->refs_dev.ptr, n, t->scratch,
                       t->bvh);
+}
+
+// SOLHIP_PROG_PATHTRACE: buffers, emitter table, the wavefront passes
+int trace_wavefront(Context *c, hipStream_t s, Program *p, Tlas *t, const ProgArgs &a)
+{
+    WfArgs w;
+    std::memset(&w, 0, sizeof(w));
+    w.sv = a.sv;
+    w.tb = a.tb;
+    w.W = a.W;
+    w.H = a.H;
+    w.accum = a.accum;
+    w.out_rgba8 = a.out_rgba8;
+    w.enable_sky = p->spec.enable_sky;
+    w.spp = p->spec.spp_per_frame;
+    w.max_depth = p->spec.max_depth;
+    w.rr_start = p->spec.rr_start;
+    w.nee = p->spec.nee;
+    w.rmax = 10000.0f;   // pathtrace.rgen:35
+    const uint64_t tiles = (uint64_t)((a.W + 7) / 8) * ((a.H + 7) / 8);
+    if (tiles * 64 > 0xFFFFFFF0ull) return bad_arg("image too large");
+    w.nslots = (uint32_t)(tiles * 64);
+    w.block = p->spec.wf_block ? p->spec.wf_block : 64u;
+    if ((uint64_t)w.sv.stack_depth * sizeof(uint2) * w.block > 65536) {
+        set_error("traversal stack of the wavefront kernels exceeds 64 KiB of LDS; use a smaller wf_block");
+        return SOLHIP_ERROR_UNSUPPORTED;
+    }
+    const uint32_t per_cu = p->spec.wf_blocks_per_cu ? p->spec.wf_blocks_per_cu : std::max(1u, 1024u / w.block);
+    w.persist_blocks = c->ncu * per_cu;
+    w.shade_blocks = (uint32_t)std::min<uint64_t>((w.nslots + w.block - 1) / w.block, (uint64_t)c->ncu * 2048 / w.block);
+    w.counters = c->counters;
+    w.count = a.count;
+    const uint64_t npix = (uint64_t)a.W * a.H, ns = w.nslots;
+    const uint64_t nheads = 2ull * (w.max_depth + 1);
+    if (p->st0.reserve(npix * 16) || p->st1.reserve(npix * 16) || p->psum.reserve(npix * 16) ||
+        p->q0.reserve(ns * sizeof(QRay)) || p->q1.reserve(ns * sizeof(QRay)) || p->hits.reserve(ns * sizeof(solhip_hit)) ||
+        p->sq.reserve(ns * sizeof(QRay)) || p->scontrib.reserve(ns * 16) || p->counts.reserve(256) ||
+        p->heads.reserve(nheads * 4) || p->fc.reserve(sizeof(FrameConsts)))
+        return SOLHIP_ERROR_OUT_OF_MEMORY;
+    FrameConsts fc;
+    std::memset(&fc, 0, sizeof(fc));
+    fc.cam = a.cam;
+    fc.frame = a.frame;
+    fc.alpha = a.alpha;
+    w.fc = (const FrameConsts *)p->fc.ptr;
+    w.st0 = (float4 *)p->st0.ptr;
+    w.st1 = (float4 *)p->st1.ptr;
+    w.psum = (float4 *)p->psum.ptr;
+    w.q[0] = (QRay *)p->q0.ptr;
+    w.q[1] = (QRay *)p->q1.ptr;
+    w.hits = (solhip_hit *)p->hits.ptr;
+    w.sq = (QRay *)p->sq.ptr;
+    w.scontrib = (float4 *)p->scontrib.ptr;
+    w.counts = (uint32_t *)p->counts.ptr;
+    w.heads = (uint32_t *)p->heads.ptr;
+    Timer *timer = c->timer_or_null();
+    int rc0 = launch_set_frame(s, (FrameConsts *)p->fc.ptr, fc);
+    if (rc0) return rc0;
+    if (w.nee) {
+        // emitter table: counts and offsets on the device, one read-back of
+        // the total to size the table
+        const uint32_t n_inst = std::min(t->n_inst, a.tb.instance_count);
+        if (p->light_offsets.reserve(((uint64_t)n_inst + 1) * 4)) return SOLHIP_ERROR_OUT_OF_MEMORY;
+        uint32_t *offs = (uint32_t *)p->light_offsets.ptr;
+        int rc = launch_lights_count(s, a.tb, (const uint32_t *)t->inst_tris_dev.ptr, n_inst, offs, timer);
+        if (rc) return rc;
+        uint32_t nl = 0;
+        SOLHIP_HIP_CHECK(hipMemcpyAsync(&nl, offs + n_inst, 4, hipMemcpyDeviceToHost, s));
+        SOLHIP_HIP_CHECK(hipStreamSynchronize(s));
+        if (nl) {
+            if (p->lights.reserve((uint64_t)nl * sizeof(LightTri))) return SOLHIP_ERROR_OUT_OF_MEMORY;
+            rc = launch_lights_fill(s, a.tb, offs, n_inst, (LightTri *)p->lights.ptr, nl, timer);
+            if (rc) return rc;
+        }
+        w.lights = (const LightTri *)p->lights.ptr;
+        w.nlights = nl;
+    }
+    if (!p->spec.wf_graph || timer) return launch_pathtrace(s, w, timer);
+    // replay the captured passes; recapture when anything the kernels were
+    // launched with differs (buffers, scene pointers, sizes, spec)
+    if (!p->gvalid || std::memcmp(&p->gkey, &w, sizeof(w)) != 0) {
+        p->drop_graph();
+        if (!p->cap_stream) SOLHIP_HIP_CHECK(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+        SOLHIP_HIP_CHECK(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+        const int rc = launch_pathtrace(p->cap_stream, w, nullptr);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(p->cap_stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ec != hipSuccess || !g) {
+            set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+            return SOLHIP_ERROR_DEVICE;
+        }
+        p->graph = g;
+        if (hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0) != hipSuccess) {
+            p->drop_graph();
+            set_error("hipGraphInstantiate failed");
+            return SOLHIP_ERROR_DEVICE;
+        }
+        p->gkey = w;
+        p->gvalid = true;
+        p->graph_captures++;
+    }
+    SOLHIP_HIP_CHECK(hipGraphLaunch(p->gexec, s));
+    return SOLHIP_OK;
 }
 
 }  // namespace solhip
@@ -274,6 +416,7 @@ int32_t solhip_context_create(int32_t device, uint32_t flags, solhip_ctx *out)
     Context *c = new Context();
     c->kind = Kind::Context;
     c->device = device;
+    if (prop.multiProcessorCount > 0) c->ncu = (uint32_t)prop.multiProcessorCount;
     if (hipStreamCreate(&c->stream) != hipSuccess ||
         hipMalloc(&c->counters, sizeof(Counters) * SOLHIP_KERNEL_COUNT) != hipSuccess) {
         set_error("context allocation failed");
@@ -628,6 +771,10 @@ void solhip_spec_default(uint32_t kind, solhip_spec *spec)
     spec->ao_tmin = 2e-4f;
     spec->ao_tmax = 0.2f;
     spec->ao_any_hit = 0;
+    spec->rr_start = 0;
+    spec->nee = 0;
+    spec->wf_blocks_per_cu = 0;
+    spec->wf_block = 0;
 }
 
 int32_t solhip_program_create(solhip_ctx ctx, uint32_t kind, const solhip_spec *spec, solhip_program *out)
@@ -635,7 +782,7 @@ int32_t solhip_program_create(solhip_ctx ctx, uint32_t kind, const solhip_spec *
     Context *c = ctx_of(ctx);
     if (!c) return bad_handle("context");
     if (!out) return bad_arg("out");
-    if (kind < SOLHIP_PROG_AO_GLSL || kind > SOLHIP_PROG_AO) return bad_arg("program kind");
+    if (kind < SOLHIP_PROG_AO_GLSL || kind > SOLHIP_PROG_PATHTRACE) return bad_arg("program kind");
     Program *p = new Program();
     p->kind = Kind::Program;
     p->ctx = ctx;
@@ -652,9 +799,17 @@ int32_t solhip_program_create(solhip_ctx ctx, uint32_t kind, const solhip_spec *
             return SOLHIP_ERROR_UNSUPPORTED;
         }
     }
-    if (kind == SOLHIP_PROG_PATHTRACE_GLSL && (p->spec.spp_per_frame == 0)) {
+    if ((kind == SOLHIP_PROG_PATHTRACE_GLSL || kind == SOLHIP_PROG_PATHTRACE) && (p->spec.spp_per_frame == 0)) {
         delete p;
         return bad_arg("spp_per_frame must be > 0");
+    }
+    if (kind == SOLHIP_PROG_PATHTRACE) {
+        const uint32_t blk = p->spec.wf_block;
+        if (p->spec.max_depth > 4096 || p->spec.enable_sky > 2 || (blk && (blk % 64 || blk > 256))) {
+            delete p;
+            set_error("pathtrace spec: max_depth <= 4096, enable_sky in {0,1,2}, wf_block a multiple of 64 <= 256");
+            return SOLHIP_ERROR_UNSUPPORTED;
+        }
     }
     *out = reg(p);
     return SOLHIP_OK;
@@ -667,8 +822,7 @@ int32_t solhip_program_destroy(solhip_ctx ctx, solhip_program ph)
     Program *p = static_cast<Program *>(unreg(ph, Kind::Program));
     if (!p) return bad_handle("program");
     (void)hipDeviceSynchronize();
-    p->queue.release();
-    p->qcount.release();
+    p->release_all();
     delete p;
     return SOLHIP_OK;
 }
@@ -697,7 +851,8 @@ int32_t solhip_trace(solhip_ctx ctx, solhip_stream st, solhip_program ph, solhip
         if (!tables || !tables->instances_addr || !tables->vertex_table_addr || !tables->index_table_addr)
             return bad_arg("scene tables");
         if (tables->index_size != 4 && tables->index_size != 8) return bad_arg("index_size must be 4 or 8");
-        if (p->kind_id == SOLHIP_PROG_PATHTRACE_GLSL && !tables->material_table_addr)
+        if ((p->kind_id == SOLHIP_PROG_PATHTRACE_GLSL || p->kind_id == SOLHIP_PROG_PATHTRACE) &&
+            !tables->material_table_addr)
             return bad_arg("pathtrace needs the material table");
     }
     if (stack_depth_of(t) > (uint32_t)kMaxStack) {
@@ -768,8 +923,112 @@ int32_t solhip_trace(solhip_ctx ctx, solhip_stream st, solhip_program ph, solhip
     case SOLHIP_PROG_AO_GLSL: return launch_ao_glsl(s, a, c->timer_or_null());
     case SOLHIP_PROG_PATHTRACE_GLSL: return launch_pathtrace_glsl(s, a, c->timer_or_null());
     case SOLHIP_PROG_DEBUG: return launch_debug(s, a, c->timer_or_null());
+    case SOLHIP_PROG_PATHTRACE: return trace_wavefront(c, s, p, t, a);
     default: return bad_arg("program kind");
     }
+}
+
+int32_t solhip_pathtrace_autotune(solhip_ctx ctx, solhip_stream st, solhip_tlas th, const solhip_scene_tables *tables,
+                                  const solhip_scene_uniforms *u, const solhip_images *images, uint32_t W, uint32_t H,
+                                  const solhip_spec *spec, uint32_t frames, const char *cache_path, solhip_spec *best,
+                                  float *best_ms)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    bool ok;
+    hipStream_t s = stream_of(c, st, ok);
+    if (!ok) return bad_handle("stream");
+    Tlas *t = get<Tlas>(th, Kind::Tlas);
+    if (!t) return bad_handle("TLAS");
+    if (!spec || !best || !u || !images || frames == 0) return bad_arg("spec / best / uniforms / images / frames");
+    // workload key: device, CUs, image, estimator settings, scene size
+    uint64_t tris = 0;
+    for (uint64_t h : t->blas_handles) {
+        const Blas *b = get<Blas>(h, Kind::Blas);
+        if (b) tris += b->bvh.prim_count;
+    }
+    char key[256];
+    std::snprintf(key, sizeof(key), "gfx950 cu=%u %ux%u spp=%u depth=%u nee=%u rr=%u sky=%u inst=%u blas_tris=%llu",
+                  c->ncu, W, H, spec->spp_per_frame, spec->max_depth, spec->nee, spec->rr_start, spec->enable_sky,
+                  t->n_inst, (unsigned long long)tris);
+    if (cache_path) {
+        if (FILE *f = std::fopen(cache_path, "r")) {
+            char line[512];
+            while (std::fgets(line, sizeof(line), f)) {
+                const size_t kl = std::strlen(key);
+                unsigned blk = 0, pcu = 0, gr = 0;
+                float ms = 0.0f;
+                if (std::strncmp(line, key, kl) == 0 && line[kl] == ' ' &&
+                    std::sscanf(line + kl, " -> block=%u per_cu=%u graph=%u ms=%f", &blk, &pcu, &gr, &ms) == 4) {
+                    *best = *spec;
+                    best->wf_block = blk;
+                    best->wf_blocks_per_cu = pcu;
+                    best->wf_graph = gr;
+                    if (best_ms) *best_ms = ms;
+                    std::fclose(f);
+                    return SOLHIP_OK;
+                }
+            }
+            std::fclose(f);
+        }
+    }
+    const uint32_t saved_profile = c->profile;
+    c->profile = 0;   // per-launch events would defeat the graph
+    const uint32_t depth = stack_depth_of(t);
+    float best_t = 1e30f;
+    solhip_spec winner = *spec;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = SOLHIP_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = SOLHIP_ERROR_DEVICE;
+    for (uint32_t blk : {64u, 128u, 256u}) {
+        if ((uint64_t)depth * 8 * blk > 65536) continue;
+        const uint32_t base = 1024u / blk;
+        for (uint32_t pcu : {base / 2 ? base / 2 : 1u, base, 2 * base}) {
+            for (uint32_t gr : {0u, 1u}) {
+                if (rc) break;
+                solhip_spec cand = *spec;
+                cand.wf_block = blk;
+                cand.wf_blocks_per_cu = pcu;
+                cand.wf_graph = gr;
+                solhip_program ph = 0;
+                rc = solhip_program_create(ctx, SOLHIP_PROG_PATHTRACE, &cand, &ph);
+                if (rc) break;
+                solhip_scene_uniforms uf = *u;
+                // one untimed frame (allocation, capture), then the timed ones
+                for (uint32_t f = 0; f <= frames && !rc; ++f) {
+                    if (f == 1) (void)hipEventRecord(e0, s);
+                    uf.frame[2] = u->frame[2] + f;
+                    rc = solhip_trace(ctx, st, ph, th, tables, &uf, (int32_t)uf.frame[2], images, W, H);
+                }
+                if (!rc) {
+                    (void)hipEventRecord(e1, s);
+                    (void)hipEventSynchronize(e1);
+                    float ms = 0.0f;
+                    (void)hipEventElapsedTime(&ms, e0, e1);
+                    ms /= (float)frames;
+                    if (ms < best_t) {
+                        best_t = ms;
+                        winner = cand;
+                    }
+                }
+                (void)solhip_program_destroy(ctx, ph);
+            }
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    c->profile = saved_profile;
+    if (rc) return rc;
+    *best = winner;
+    if (best_ms) *best_ms = best_t;
+    if (cache_path) {
+        if (FILE *f = std::fopen(cache_path, "a")) {
+            std::fprintf(f, "%s -> block=%u per_cu=%u graph=%u ms=%.4f\n", key, winner.wf_block, winner.wf_blocks_per_cu,
+                         winner.wf_graph, best_t);
+            std::fclose(f);
+        }
+    }
+    return SOLHIP_OK;
 }
 
 int32_t solhip_stats(solhip_ctx ctx, solhip_counters *out, uint32_t reset)

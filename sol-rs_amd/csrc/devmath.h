@@ -114,11 +114,10 @@ SOLHIP_HD void own_sincos(float x, float &s, float &c)
     c = ((q + 1) & 2) ? -c0 : c0;
 }
 
-// assets/glsl/sampling.glsl:52-64
-SOLHIP_HD float fresnel_dielectric(V3 i, V3 m, float eta)
+// assets/glsl/sampling.glsl:52-64, as a function of cos(theta_i)
+SOLHIP_HD float fresnel_cos(float cosThetaI, float eta)
 {
     float result = 1.0f;
-    float cosThetaI = fabsf(dot(i, m));
     float sinThetaOSquared = (eta * eta) * (1.0f - cosThetaI * cosThetaI);
     if (sinThetaOSquared <= 1.0f) {
         float cosThetaO = sqrtf(saturate(1.0f - sinThetaOSquared));
@@ -128,6 +127,9 @@ SOLHIP_HD float fresnel_dielectric(V3 i, V3 m, float eta)
     }
     return result;
 }
+
+// assets/glsl/sampling.glsl:52-64
+SOLHIP_HD float fresnel_dielectric(V3 i, V3 m, float eta) { return fresnel_cos(fabsf(dot(i, m)), eta); }
 
 // assets/glsl/sampling.glsl:66-84 (basis of Duff et al. 2017)
 SOLHIP_HD V3 align_to_direction(V3 n, float cosTheta, float phi)

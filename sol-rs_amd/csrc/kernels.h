@@ -60,6 +60,61 @@ struct ProgArgs {
     bool count;           // fetch-counting kernel variants
 };
 
+// ---- wavefront path tracer (SOLHIP_PROG_PATHTRACE) ------------------------
+
+// emitter triangle of next-event estimation (64 B): world-space vertices,
+// unit geometric normal, emission, running (inclusive) area sum
+struct alignas(16) LightTri {
+    float p0[3], p1[3], p2[3], n[3], e[3];
+    float cdf;
+};
+
+// extension / shadow queue entry (32 B): ray and the path it belongs to
+// (kNoPath marks an empty slot of the first queue)
+struct alignas(16) QRay {
+    float ox, oy, oz;
+    uint32_t path;
+    float dx, dy, dz;
+    float tmax;
+};
+constexpr uint32_t kNoPath = 0xFFFFFFFFu;
+
+// per-frame constants of the wavefront kernels, in device memory so that a
+// captured frame can be replayed with new uniforms
+struct FrameConsts {
+    Cam cam;
+    uint32_t frame;
+    float alpha;
+    uint32_t pad[2];
+};
+
+struct WfArgs {
+    SceneView sv;
+    DevTables tb;
+    const FrameConsts *fc;
+    uint32_t W, H;
+    float4 *accum;
+    uint32_t *out_rgba8;
+    uint32_t enable_sky, spp, max_depth, rr_start, nee;
+    float rmin, rmax;                 // ray range of every segment (pathtrace.rgen:35)
+    uint32_t nslots;                  // 8x8 pixel tiles * 64: slots of the first queue
+    // per-pixel path state: (T, rng), (L, depth | count_emission << 16), pixel sum
+    float4 *st0, *st1, *psum;
+    QRay *q[2];                       // extension queues (ping-pong by bounce)
+    solhip_hit *hits;                 // closest hits, by slot of the current queue
+    QRay *sq;                         // shadow queue
+    float4 *scontrib;                 // shadow-ray contributions, by shadow slot
+    uint32_t *counts;                 // [0], [1]: extension queue sizes, [2]: shadow queue size
+    uint32_t *heads;                  // dynamic-fetch heads, 2 per bounce
+    const LightTri *lights;
+    uint32_t nlights;
+    uint32_t block;                   // threads per block
+    uint32_t persist_blocks;          // grid of the extend / shadow kernels
+    uint32_t shade_blocks;            // grid of the shade kernel
+    Counters *counters;
+    bool count;
+};
+
 int launch_trace_rays(hipStream_t s, const SceneView &sv, const solhip_ray *rays, solhip_hit *hits, uint32_t n,
                       uint32_t flags, Counters *counters, Timer *timer);
 int launch_ao(hipStream_t s, const ProgArgs &a, Timer *timer);
@@ -67,5 +122,18 @@ int launch_ao_glsl(hipStream_t s, const ProgArgs &a, Timer *timer);
 int launch_pathtrace_glsl(hipStream_t s, const ProgArgs &a, Timer *timer);
 int launch_debug(hipStream_t s, const ProgArgs &a, Timer *timer);
 int launch_fill(hipStream_t s, uint32_t *p, uint64_t nwords, uint32_t value);
+
+// emitter table: per-instance triangle counts of emissive instances, their
+// exclusive scan (n_inst + 1 entries, the last one the total), then the
+// records and the running area sum.  lights_count needs no host round trip;
+// the caller reads offsets[n_inst] before lights_fill.
+int launch_lights_count(hipStream_t s, const DevTables &tb, const uint32_t *inst_tris, uint32_t n_inst,
+                        uint32_t *offsets, Timer *timer);
+int launch_lights_fill(hipStream_t s, const DevTables &tb, const uint32_t *offsets, uint32_t n_inst,
+                       LightTri *lights, uint32_t nlights, Timer *timer);
+// one frame of the wavefront path tracer (kernels only: graph-capturable)
+int launch_pathtrace(hipStream_t s, const WfArgs &a, Timer *timer);
+// writes the frame constants through a kernel argument (no host buffer to keep alive)
+int launch_set_frame(hipStream_t s, FrameConsts *dst, const FrameConsts &value);
 
 }  // namespace solhip

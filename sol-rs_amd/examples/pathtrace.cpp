@@ -5,6 +5,11 @@
 //
 //   pathtrace --model scene.gltf [--sky] [--size 1280x720] [--frames 16]
 //             [--out image.ppm] [--accum accum.f32] [--dump dir] [--device 0]
+//             [--wavefront [--nee] [--rr N] [--spp N] [--depth N]]
+//
+// --wavefront runs the wavefront path tracer (same image as the default
+// program without --nee / --rr); --nee adds next-event estimation, --rr N
+// Russian roulette from bounce N.
 #include <chrono>
 
 #include "common.hpp"
@@ -29,12 +34,23 @@ int main(int argc, char **argv)
         ray::SceneDescription sd = ray::SceneDescription::from_scene(ctx, sc);
         scene::Camera camera = sc.camera ? *sc.camera : scene::Camera(Vec2{(float)W, (float)H});
         camera.set_window_size(Vec2{(float)W, (float)H});
-        ray::Pipeline pipeline(ctx, ray::PipelineInfo()
-                                        .shader("glsl/pathtrace.rgen", ray::ShaderStage::Raygen)
-                                        .shader("glsl/pathtrace.rmiss", ray::ShaderStage::Miss)
-                                        .shader("glsl/pathtrace.rchit", ray::ShaderStage::ClosestHit)
-                                        .specialization(enable_sky ? 1u : 0u, 0)
-                                        .name("AO_mat"));
+        const bool wavefront = example::has_flag(argc, argv, "--wavefront");
+        solhip_spec spec;
+        solhip_spec_default(wavefront ? SOLHIP_PROG_PATHTRACE : SOLHIP_PROG_PATHTRACE_GLSL, &spec);
+        spec.enable_sky = enable_sky ? 1u : 0u;
+        spec.nee = example::has_flag(argc, argv, "--nee") ? 1u : 0u;
+        spec.rr_start = (uint32_t)std::atoi(example::arg(argc, argv, "--rr", "0").c_str());
+        spec.spp_per_frame = (uint32_t)std::atoi(example::arg(argc, argv, "--spp", "8").c_str());
+        spec.max_depth = (uint32_t)std::atoi(example::arg(argc, argv, "--depth", "32").c_str());
+        ray::Pipeline pipeline(ctx,
+                               ray::PipelineInfo()
+                                   .shader(wavefront ? "glsl/pathtrace_wavefront.rgen" : "glsl/pathtrace.rgen",
+                                           ray::ShaderStage::Raygen)
+                                   .shader("glsl/pathtrace.rmiss", ray::ShaderStage::Miss)
+                                   .shader("glsl/pathtrace.rchit", ray::ShaderStage::ClosestHit)
+                                   .specialization(enable_sky ? 1u : 0u, 0)
+                                   .name("AO_mat"),
+                               &spec);
         ray::ShaderBindingTable sbt(ctx, pipeline, ray::ShaderBindingTableInfo().raygen(0).miss(1).hitgroup(2));
         Buffer accum(ctx, (uint64_t)W * H * 16), render(ctx, (uint64_t)W * H * 4);
         std::vector<float> zeros((size_t)W * H * 4, 0.0f);

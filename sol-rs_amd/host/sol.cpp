@@ -429,10 +429,13 @@ bool ends_with(const std::string &s, const std::string &suffix)
 Pipeline::Pipeline(std::shared_ptr<Context> ctx, const PipelineInfo &info, const solhip_spec *override_spec)
     : ctx_(std::move(ctx))
 {
-    // the raygen shader names the program (assets/glsl/{ao,pathtrace,debug}.rgen)
+    // the raygen shader names the program (assets/glsl/{ao,pathtrace,debug}.rgen;
+    // "pathtrace_wavefront.rgen" selects the wavefront form of pathtrace.*,
+    // whose extensions come in override_spec)
     for (const auto &s : info.shaders) {
         if (s.second != ShaderStage::Raygen) continue;
-        if (ends_with(s.first, "pathtrace.rgen")) kind_ = SOLHIP_PROG_PATHTRACE_GLSL;
+        if (ends_with(s.first, "pathtrace_wavefront.rgen")) kind_ = SOLHIP_PROG_PATHTRACE;
+        else if (ends_with(s.first, "pathtrace.rgen")) kind_ = SOLHIP_PROG_PATHTRACE_GLSL;
         else if (ends_with(s.first, "ao.rgen")) kind_ = SOLHIP_PROG_AO_GLSL;
         else if (ends_with(s.first, "debug.rgen")) kind_ = SOLHIP_PROG_DEBUG;
         else if (ends_with(s.first, "ao_plain.rgen")) kind_ = SOLHIP_PROG_AO;

@@ -14,7 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("SOLHIP_LIB") or os.path.normpath(os.path.join(_HERE, "..", "..", "lib", "libsolhip.so"))
 HEADER_PATH = os.path.normpath(os.path.join(_HERE, "..", "..", "..", "include", "solhip.h"))
 
-PROG_AO_GLSL, PROG_PATHTRACE_GLSL, PROG_DEBUG, PROG_AO = 1, 2, 3, 4
+PROG_AO_GLSL, PROG_PATHTRACE_GLSL, PROG_DEBUG, PROG_AO, PROG_PATHTRACE = 1, 2, 3, 4, 5
 TRACE_TERMINATE_ON_FIRST_HIT = 0x4
 TRACE_COUNT = 0x100
 
@@ -43,7 +43,8 @@ class SceneTables(C.Structure):
 class Spec(C.Structure):
     _fields_ = [("enable_sky", C.c_uint32), ("spp_per_frame", C.c_uint32), ("max_depth", C.c_uint32),
                 ("n_ao", C.c_uint32), ("ao_tmin", C.c_float), ("ao_tmax", C.c_float), ("ao_any_hit", C.c_uint32),
-                ("reserved", C.c_uint32 * 9)]
+                ("rr_start", C.c_uint32), ("nee", C.c_uint32), ("wf_blocks_per_cu", C.c_uint32),
+                ("wf_block", C.c_uint32), ("wf_graph", C.c_uint32), ("reserved", C.c_uint32 * 4)]
 
 
 class Images(C.Structure):
@@ -60,7 +61,8 @@ class KernelStats(C.Structure):
     _fields_ = [("launches", C.c_uint64), ("total_ms", C.c_double), ("counters", Counters)]
 
 
-KERNELS = {"trace_rays": 0, "ao_primary": 1, "ao_rays": 2, "ao_glsl": 3, "pathtrace_glsl": 4, "debug": 5}
+KERNELS = {"trace_rays": 0, "ao_primary": 1, "ao_rays": 2, "ao_glsl": 3, "pathtrace_glsl": 4, "debug": 5,
+           "pt_generate": 6, "pt_extend": 7, "pt_shade": 8, "pt_shadow": 9, "pt_finalize": 10, "pt_lights": 11}
 PROFILE_TIME, PROFILE_COUNT = 0x1, 0x2
 
 
@@ -101,6 +103,9 @@ SIGNATURES = {
     "solhip_stats": (_i32, [_u64, C.POINTER(Counters), _u32]),
     "solhip_profile": (_i32, [_u64, _u32]),
     "solhip_kernel_stats_get": (_i32, [_u64, _u32, C.POINTER(KernelStats), _u32]),
+    "solhip_pathtrace_autotune": (_i32, [_u64, _u64, _u64, C.POINTER(SceneTables), _vp, C.POINTER(Images), _u32, _u32,
+                                         C.POINTER(Spec), _u32, C.c_char_p, C.POINTER(Spec),
+                                         C.POINTER(C.c_float)]),
 }
 
 _lib = None
@@ -275,6 +280,16 @@ class Context:
 
     def profile(self, flags):
         check(lib().solhip_profile(self.h, flags))
+
+    def autotune_pathtrace(self, tlas, tables, uniforms, images, W, H, spec, frames=3, cache_path=None, stream=0):
+        """fastest wavefront launch shape for this workload: (spec, ms per frame)"""
+        u = np.ascontiguousarray(uniforms, UNIFORMS)
+        best = Spec()
+        ms = C.c_float(0.0)
+        path = cache_path.encode() if cache_path else None
+        check(lib().solhip_pathtrace_autotune(self.h, stream, tlas, C.byref(tables), _ptr(u), C.byref(images), W, H,
+                                              C.byref(spec), frames, path, C.byref(best), C.byref(ms)))
+        return best, float(ms.value)
 
     def kernel_stats(self, kernel, reset=False):
         k = KERNELS[kernel] if isinstance(kernel, str) else kernel

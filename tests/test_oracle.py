@@ -270,3 +270,117 @@ def test_accumulation_is_a_running_mean(oracle):
         singles.append(one[..., 0].astype(np.float64))
         sc.render_ao(sd.uniforms(W, H, f), oracle.AOParams(8, 2e-4, 0.2, 0), 0, acc, W, H)
     assert np.allclose(acc[..., 0], np.mean(singles, axis=0), atol=1e-6)
+
+
+# ---------------------------------------------------------------------------
+# the wavefront path tracer's estimator (next-event estimation, roulette)
+# ---------------------------------------------------------------------------
+
+def test_pathtrace_config_without_extensions_is_the_glsl_program(oracle):
+    """nee = 0, rr_start = 0: the same draws and products as pathtrace.*"""
+    for sd, sky in ((scenes.cornell_box(), 0), (scenes.random_instances(), 1), (scenes.cornell_box(), 1)):
+        sc = oracle.Scene(sd)
+        W, H = 20, 12
+        a = np.zeros((H, W, 4), np.float32)
+        b = np.zeros((H, W, 4), np.float32)
+        o1 = np.zeros((H, W, 4), np.uint8)
+        o2 = np.zeros((H, W, 4), np.uint8)
+        for f in range(2):
+            u = sd.uniforms(W, H, f)
+            n1 = sc.render_pathtrace_glsl(u, oracle.PTParams(sky, 3, 6), 0, a, o1, W, H)
+            n2 = sc.render_pathtrace(u, oracle.PTConfig(sky, 3, 6, 0, 0), 0, b, o2, W, H)
+            assert n1 == n2
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)) and np.array_equal(o1, o2)
+
+
+def test_cornell_light_table(oracle):
+    """two triangles of the 0.49 x 0.49 quad at y = 0.999: total area 0.24,
+    normals along y, emission 15, inclusive running sum"""
+    sd = scenes.cornell_box()
+    lights, total = oracle.Scene(sd).lights()
+    assert len(lights) == 2
+    assert abs(total - 0.24) < 1e-6 and abs(lights["cdf"][0] - 0.12) < 1e-6 and lights["cdf"][1] == np.float32(total)
+    assert np.allclose(np.abs(lights["n"][:, 1]), 1.0) and np.allclose(lights["n"][:, [0, 2]], 0.0)
+    assert (lights["e"] == 15.0).all()
+    assert np.allclose(lights["p0"][:, 1], 0.999)
+    # no emitter: empty table
+    l2, t2 = oracle.Scene(scenes.random_instances()).lights()
+    assert len(l2) == 0 and t2 == 0.0
+
+
+def _lit_box():
+    """closed box [-1,1]^3: white diffuse floor and walls (0.8), an emissive
+    ceiling (radiance 1.5, an emitter since 1.5 >= 1) split in 2 x 2 quads, a
+    small grey block on the floor; camera inside looking down at the floor.
+    The light covers a large solid angle, so the plain estimator is not too
+    noisy and a biased variant stands out."""
+    L, R = -1.0, 1.0
+    floor = scenes.quad_tris((L, L, L), (R, L, L), (R, L, R), (L, L, R))
+    walls = (scenes.quad_tris((L, L, L), (L, R, L), (R, R, L), (R, L, L)) +
+             scenes.quad_tris((L, L, R), (R, L, R), (R, R, R), (L, R, R)) +
+             scenes.quad_tris((L, L, L), (L, L, R), (L, R, R), (L, R, L)) +
+             scenes.quad_tris((R, L, L), (R, R, L), (R, R, R), (R, L, R)))
+    ceil = []
+    for x0 in (L, 0.0):
+        for z0 in (L, 0.0):
+            ceil += scenes.quad_tris((x0, R, z0), (x0, R, z0 + 1), (x0 + 1, R, z0 + 1), (x0 + 1, R, z0))
+    block = scenes.box_tris((0.5, 0.5, 0.5), 30.0, (0.2, -1.0, 0.1))
+    parts = [(floor + walls, scenes.material((0.8, 0.8, 0.8))), (block, scenes.material((0.5, 0.5, 0.5))),
+             (ceil, scenes.material((1, 1, 1), emissive=(1.5, 1.5, 1.5)))]
+    blas = [scenes.soup_blas(p) for p, _ in parts]
+    cam = scenes.Camera((0.0, 0.3, 0.9), (0.0, -1.0, 0.0), up=(0.0, 0.0, -1.0), vfov_deg=60.0)
+    return scenes.assemble("lit_box", blas, [0, 1, 2], [np.eye(4)] * 3, [0, 1, 2], [m for _, m in parts], cam, 2.0)
+
+
+def _frame_means(oracle, sd, cfg, W, H, frames):
+    """image mean of each independent frame (alpha = 1: accum_start = frame)"""
+    sc = oracle.Scene(sd)
+    out = []
+    for f in range(frames):
+        acc = np.zeros((H, W, 4), np.float32)
+        sc.render_pathtrace(sd.uniforms(W, H, f), cfg, f, acc, None, W, H)
+        out.append(acc[..., :3].astype(np.float64).mean(axis=(0, 1)))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("nee,rr", [(1, 0), (0, 1), (1, 2)])
+def test_nee_and_roulette_keep_the_expectation(oracle, nee, rr):
+    """next-event estimation and Russian roulette are unbiased: the image
+    mean agrees with the plain estimator within 5 standard errors (same
+    scene, same depth limit, independent seeds per frame).  At this sample
+    count a 10 % error in the next-event weight or a 10 % bias in the
+    roulette weight is far outside that band."""
+    sd = _lit_box()
+    W, H, F = 16, 16, 24
+    plain = _frame_means(oracle, sd, oracle.PTConfig(0, 32, 3, 0, 0), W, H, F)
+    ext = _frame_means(oracle, sd, oracle.PTConfig(0, 32, 3, rr, nee), W, H, F)
+    m1, m2 = plain.mean(0), ext.mean(0)
+    se = np.sqrt(plain.var(0, ddof=1) / F + ext.var(0, ddof=1) / F)
+    assert (np.abs(m1 - m2) < 5 * se).all(), (m1, m2, se)
+    assert (se < 0.01 * m1).all()   # the band is tight
+
+
+def test_nee_reduces_noise_for_a_small_light(oracle):
+    """Cornell box (light 6 % of the ceiling), direct light only: sampling
+    the emitter beats waiting for a cosine-sampled ray to hit it"""
+    sd = scenes.cornell_box()
+    W, H, F = 16, 16, 12
+    plain = _frame_means(oracle, sd, oracle.PTConfig(0, 16, 1, 0, 0), W, H, F)
+    nee = _frame_means(oracle, sd, oracle.PTConfig(0, 16, 1, 0, 1), W, H, F)
+    assert (nee.var(0) < 0.5 * plain.var(0)).all()
+    se = np.sqrt(plain.var(0, ddof=1) / F + nee.var(0, ddof=1) / F)
+    assert (np.abs(plain.mean(0) - nee.mean(0)) < 5 * se).all()
+
+
+def test_depth_zero_sees_only_emitters_and_sky(oracle):
+    """max_bounces = 0: a path ends at its first surface unless that is an
+    emitter, so next-event estimation adds nothing"""
+    sd = scenes.cornell_box()
+    W = H = 12
+    a = np.zeros((H, W, 4), np.float32)
+    b = np.zeros((H, W, 4), np.float32)
+    oracle.Scene(sd).render_pathtrace(sd.uniforms(W, H, 0), oracle.PTConfig(0, 4, 0, 0, 0), 0, a, None, W, H)
+    oracle.Scene(sd).render_pathtrace(sd.uniforms(W, H, 0), oracle.PTConfig(0, 4, 0, 0, 1), 0, b, None, W, H)
+    assert np.array_equal(a, b)
+    vals = np.unique(a[..., 0])
+    assert set(np.round(vals * 4).astype(int)) <= {0, 15, 30, 45, 60}   # k of 4 samples saw the light
