@@ -20,6 +20,15 @@ its own binary BVH, OpenMP over rows) on whole frames of the same workload,
 as many as fit ~10 s; the GPU accumulation of the same frames is compared
 with it bit for bit.
 
+    python bench.py --workload pathtrace-c0|pathtrace-c2|server-c3 [--steps K] [--warmup W]
+
+other configurations of the path, one JSON line each (not the headline
+metric): the wavefront path tracer on the 36-triangle Cornell box (512^2,
+16 spp, depth 4, next-event estimation) and on 10,000 instances of 5 BLAS
+(1080p, 64 spp, depth 8, Russian roulette from bounce 3, gradient sky, every
+BLAS and the TLAS rebuilt each frame); the render server under a Poisson
+request load (p50 / p99 latency beside throughput).
+
     python bench.py --copy-calibration BYTES
 
 runs one plain device-to-device copy of BYTES bytes (torch), the reference
@@ -54,6 +63,9 @@ def parse_args():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", type=int, default=0)
+    ap.add_argument("--workload", default="ao", choices=["ao", "pathtrace-c0", "pathtrace-c2", "server-c3"])
+    ap.add_argument("--jobs", type=int, default=40, help="server-c3: requests")
+    ap.add_argument("--rate", type=float, default=200.0, help="server-c3: requests per second")
     ap.add_argument("--any-hit", action="store_true",
                     help="AO rays stop at the first hit (same image; hit records are not the closest)")
     return ap.parse_args()
@@ -96,6 +108,12 @@ def main():
         print("bench.py measures one GPU only (this build has no multi-GPU path); got --gpus %d" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if a.workload.startswith("pathtrace"):
+        bench_pathtrace(a)
+        return
+    if a.workload == "server-c3":
+        bench_server(a)
+        return
     W, H = (int(v) for v in a.res.lower().split("x"))
     import solhip
     from solhip import scenes
@@ -218,6 +236,93 @@ def main():
 
     if not a.no_cpu:
         result["cpu_baseline"] = cpu_baseline(ctx, gs, sd, spec, W, H, a.cpu_seconds)
+    ctx.close()
+    print(json.dumps(result))
+
+
+def bench_pathtrace(a):
+    """the wavefront path tracer on C0 (Cornell, 512^2, 16 spp, depth 4, NEE)
+    or C2 (10k instances, 1080p, 64 spp, depth 8, RR from 3, gradient sky,
+    all BLAS and the TLAS rebuilt every frame); one JSON line"""
+    import solhip
+    from solhip import scenes
+    c0 = a.workload == "pathtrace-c0"
+    sd = scenes.cornell_box() if c0 else scenes.instanced(10000)
+    W, H = (512, 512) if c0 else (1920, 1080)
+    ctx = solhip.Context(0)
+    gs = solhip.GpuScene(ctx, sd)
+    spec = solhip.spec_default(solhip.PROG_PATHTRACE)
+    if c0:
+        spec.enable_sky, spec.spp_per_frame, spec.max_depth, spec.nee, spec.rr_start = 0, 16, 4, 1, 0
+    else:
+        spec.enable_sky, spec.spp_per_frame, spec.max_depth, spec.nee, spec.rr_start = 2, 64, 8, 1, 3
+    spec.wf_graph = 1
+    prog = ctx.program(solhip.PROG_PATHTRACE, spec)
+    acc = ctx.upload(np.zeros((H, W, 4), np.float32))
+    img = solhip.Images(acc.h, 0, 0, 0, 0, 0)
+
+    def rebuild():
+        # C2: every BLAS and the TLAS again (a per-frame animated scene)
+        for k, b in enumerate(sd.blas):
+            vb, ib = gs.bufs[2 * k], gs.bufs[2 * k + 1]
+            ctx.blas_rebuild(gs.blas[k], vb, 64, len(b.vertices), ib, len(b.indices))
+        gs.update_instances(sd.instance_descs)
+
+    for i in range(a.warmup):
+        if not c0:
+            rebuild()
+        ctx.trace(prog, gs.tlas, gs.tables, sd.uniforms(W, H, i), 0, img, W, H)
+    ctx.sync()
+    ctx.stats(reset=True)
+    unis = [sd.uniforms(W, H, a.warmup + i) for i in range(a.steps)]
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        if not c0:
+            rebuild()
+        ctx.trace(prog, gs.tlas, gs.tables, unis[i], a.warmup, img, W, H)
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    st = ctx.stats(reset=True)
+    assert st.overflow == 0
+    result = {
+        "metric": "Mrays/s on 1 GPU", "value": round(st.rays / elapsed / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(elapsed / a.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": ("wavefront path tracer, Cornell box (36 triangles), 512x512, 16 spp, depth 4, "
+                                "next-event estimation" if c0 else
+                                "wavefront path tracer, 10000 instances of 5 BLAS (%d triangles), 1920x1080, 64 spp, "
+                                "depth 8, Russian roulette from bounce 3, next-event estimation (no emitters), "
+                                "gradient sky, all BLAS + TLAS rebuilt per frame" % sd.tri_count),
+                   "resolution": [W, H], "spp": spec.spp_per_frame, "max_depth": spec.max_depth,
+                   "parallelism": "single GPU"},
+        "extra": {"rays_per_frame": int(st.rays / a.steps), "hipgraph": True},
+    }
+    ctx.close()
+    print(json.dumps(result))
+
+
+def bench_server(a):
+    """Poisson request load on the render server over the C0, C1 and C2
+    scenes; p50 / p99 latency and throughput; one JSON line"""
+    import solhip
+    from solhip import scenes, serve
+    ctx = solhip.Context(0)
+    sds = [scenes.cornell_box(), scenes.heightfield(1024), scenes.instanced(10000)]
+    gss = [solhip.GpuScene(ctx, sd) for sd in sds]
+    ctx.sync()
+    warm = serve.draw_requests(4, 1000.0, seed=1, resolutions=[(256, 256)], spp_max=4)
+    serve.run_load(ctx, gss, sds, warm)
+    reqs = serve.draw_requests(a.jobs, a.rate, seed=3)
+    stats = serve.run_load(ctx, gss, sds, reqs)
+    result = {
+        "metric": "render-request latency on 1 GPU", "value": round(stats["latency_ms_p50"], 2), "unit": "ms (p50)",
+        "n_gpus": 1, "steps": a.jobs, "warmup": 4, "ms_per_step": round(stats["makespan_s"] * 1e3 / a.jobs, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "render server, Poisson %.0f req/s, %d requests over Cornell / 2.1M-triangle "
+                               "heightfield / 10k instances, resolutions 256^2..1080p, spp log-uniform 1-256, "
+                               "depth 4-8, NEE + roulette" % (a.rate, a.jobs), "parallelism": "single GPU"},
+        "extra": stats,
+    }
     ctx.close()
     print(json.dumps(result))
 

@@ -41,6 +41,7 @@ typedef uint64_t solhip_buffer;
 typedef uint64_t solhip_blas;
 typedef uint64_t solhip_tlas;
 typedef uint64_t solhip_program;
+typedef uint64_t solhip_server;
 
 enum {
     SOLHIP_OK = 0,
@@ -262,7 +263,12 @@ enum {
     SOLHIP_KERNEL_PT_SHADOW = 9,    /* any-hit shadow rays of next-event estimation */
     SOLHIP_KERNEL_PT_FINALIZE = 10, /* sample mean, accumulation, RGBA8 */
     SOLHIP_KERNEL_PT_LIGHTS = 11,   /* emitter table (next-event estimation) */
-    SOLHIP_KERNEL_COUNT = 12
+    SOLHIP_KERNEL_SRV_ADMIT = 12,   /* render server: first camera rays of an admitted job */
+    SOLHIP_KERNEL_SRV_EXTEND = 13,  /* render server: closest hits of all live jobs' rays */
+    SOLHIP_KERNEL_SRV_SHADE = 14,
+    SOLHIP_KERNEL_SRV_SHADOW = 15,
+    SOLHIP_KERNEL_SRV_RETIRE = 16,  /* render server: finished samples, next samples, finished pixels */
+    SOLHIP_KERNEL_COUNT = 17
 };
 #define SOLHIP_PROFILE_TIME   0x1u  /* hipEvents around every launch, on the launch stream */
 #define SOLHIP_PROFILE_COUNT  0x2u  /* fetch-counting kernel variants */
@@ -288,6 +294,48 @@ int32_t solhip_pathtrace_autotune(solhip_ctx ctx, solhip_stream s, solhip_tlas t
                                   const solhip_scene_uniforms *uniforms, const solhip_images *images, uint32_t W,
                                   uint32_t H, const solhip_spec *spec, uint32_t frames, const char *cache_path,
                                   solhip_spec *best, float *best_ms);
+
+/* ---- render-request server (no counterpart in the reference, which
+ * renders one image per command buffer) -----------------------------------
+ * Jobs of mixed scene, size and sample count share one wavefront: every step
+ * runs extend / shade / shadow / retire over the rays of all live jobs; a
+ * path that ends starts its pixel's next sample in place, and a pixel whose
+ * last sample ends is accumulated into its job's image.  Scenes (BVHs,
+ * tables, emitter tables) stay resident; each job holds W*H path slots while
+ * it runs.  Jobs are admitted first-come first-served while their path state
+ * fits the HBM budget given at creation.  A job's image is bit-identical to
+ * solhip_trace of SOLHIP_PROG_PATHTRACE with the same spec and uniforms. */
+typedef struct {
+    uint32_t scene;                  /* from solhip_server_scene */
+    uint32_t W, H;
+    solhip_spec spec;                /* SOLHIP_PROG_PATHTRACE settings (launch-shape fields ignored) */
+    solhip_scene_uniforms uniforms;
+    int32_t accum_start_frame;
+    uint32_t reserved;
+    solhip_buffer accum;             /* RGBA32F W*H, read-modify-write */
+    solhip_buffer out_rgba8;         /* RGBA8 W*H, or 0 */
+} solhip_job_desc;
+
+enum { SOLHIP_JOB_QUEUED = 0, SOLHIP_JOB_RUNNING = 1, SOLHIP_JOB_DONE = 2 };
+typedef struct {
+    uint32_t state;
+    uint32_t reserved;
+    double submit_ms, admit_ms, done_ms;   /* server clock (ms since creation) */
+} solhip_job_status;
+
+/* hbm_budget: bytes of path state and queues (about 190 B per pixel of the
+ * running jobs); max_jobs <= 256 running at once */
+int32_t solhip_server_create(solhip_ctx ctx, uint64_t hbm_budget, uint32_t max_jobs, solhip_server *out);
+int32_t solhip_server_destroy(solhip_ctx ctx, solhip_server srv);
+/* registers a resident scene: TLAS + hit-shader tables (+ its emitter table) */
+int32_t solhip_server_scene(solhip_ctx ctx, solhip_server srv, solhip_tlas t, const solhip_scene_tables *tables,
+                            uint32_t *scene_id);
+int32_t solhip_server_submit(solhip_ctx ctx, solhip_server srv, const solhip_job_desc *job, uint64_t *job_id);
+/* runs up to max_steps wavefront steps (admitting and retiring jobs between
+ * steps); returns early when nothing is queued or running; *live = jobs
+ * queued or running afterwards */
+int32_t solhip_server_run(solhip_ctx ctx, solhip_server srv, uint32_t max_steps, uint32_t *live);
+int32_t solhip_server_job(solhip_ctx ctx, solhip_server srv, uint64_t job_id, solhip_job_status *out);
 
 #ifdef __cplusplus
 }

@@ -15,6 +15,7 @@
 
 #include "bvh.h"
 #include "common.h"
+#include "objects.h"
 #include "kernels.h"
 #include "traverse.h"
 
@@ -22,122 +23,6 @@ namespace solhip {
 
 thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
-
-enum class Kind { Context, Stream, Buffer, Blas, Tlas, Program };
-
-struct Obj {
-    Kind kind;
-    uint64_t ctx;
-    virtual ~Obj() = default;
-};
-
-// hipEvent pairs around launches, resolved when statistics are read
-struct EventTimer : Timer {
-    struct Rec {
-        int kernel;
-        hipEvent_t a, b;
-    };
-    std::vector<Rec> pending;
-    std::vector<Rec> open;
-    void begin(hipStream_t s, int kernel) override
-    {
-        Rec r{kernel, nullptr, nullptr};
-        if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return;
-        (void)hipEventRecord(r.a, s);
-        open.push_back(r);
-    }
-    void end(hipStream_t s, int kernel) override
-    {
-        for (size_t i = open.size(); i-- > 0;) {
-            if (open[i].kernel != kernel) continue;
-            (void)hipEventRecord(open[i].b, s);
-            pending.push_back(open[i]);
-            open.erase(open.begin() + (long)i);
-            return;
-        }
-    }
-};
-
-struct Context : Obj {
-    int device = 0;
-    uint32_t ncu = 256;   // compute units (persistent grids)
-    hipStream_t stream = nullptr;
-    Counters *counters = nullptr;  // [SOLHIP_KERNEL_COUNT]
-    uint32_t profile = 0;
-    EventTimer timer;
-    uint64_t launches[SOLHIP_KERNEL_COUNT] = {};
-    double total_ms[SOLHIP_KERNEL_COUNT] = {};
-    Timer *timer_or_null() { return (profile & SOLHIP_PROFILE_TIME) ? &timer : nullptr; }
-    void resolve_events()
-    {
-        for (auto &r : timer.pending) {
-            float ms = 0.0f;
-            if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess && r.kernel >= 0 && r.kernel < SOLHIP_KERNEL_COUNT) {
-                launches[r.kernel] += 1;
-                total_ms[r.kernel] += ms;
-            }
-            (void)hipEventDestroy(r.a);
-            (void)hipEventDestroy(r.b);
-        }
-        timer.pending.clear();
-    }
-};
-
-struct Stream : Obj {
-    hipStream_t s = nullptr;
-};
-
-struct Buffer : Obj {
-    void *ptr = nullptr;
-    uint64_t bytes = 0;
-};
-
-struct Blas : Obj {
-    DeviceBvh bvh;
-    DevScratch scratch;
-};
-
-struct Tlas : Obj {
-    DeviceBvh bvh;
-    DevScratch scratch;
-    DevBuf inst_dev;   // solhip_instance[]
-    DevBuf refs_dev;   // BlasRef[]
-    std::vector<uint64_t> blas_handles;   // distinct BLAS referenced by the instances
-    uint32_t n_inst = 0;
-    DevBuf inst_tris_dev;                 // u32 triangle count of each instance's BLAS
-};
-
-struct Program : Obj {
-    uint32_t kind_id = 0;
-    solhip_spec spec{};
-    DevBuf queue, qcount;
-    // SOLHIP_PROG_PATHTRACE: path state, queues, emitter table, frame constants
-    DevBuf st0, st1, psum, q0, q1, hits, sq, scontrib, counts, heads, lights, light_offsets, fc;
-    // captured frame (wf_graph): valid for the WfArgs it was captured with
-    hipStream_t cap_stream = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    WfArgs gkey{};
-    bool gvalid = false;
-    uint64_t graph_captures = 0;
-    void drop_graph()
-    {
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        if (graph) (void)hipGraphDestroy(graph);
-        gexec = nullptr;
-        graph = nullptr;
-        gvalid = false;
-    }
-    void release_all()
-    {
-        drop_graph();
-        if (cap_stream) (void)hipStreamDestroy(cap_stream);
-        cap_stream = nullptr;
-        for (DevBuf *b : {&queue, &qcount, &st0, &st1, &psum, &q0, &q1, &hits, &sq, &scontrib, &counts, &heads,
-                          &lights, &light_offsets, &fc})
-            b->release();
-    }
-};
 
 std::mutex g_mu;
 std::unordered_map<uint64_t, Obj *> g_objs;
@@ -149,15 +34,6 @@ uint64_t reg(Obj *o)
     const uint64_t h = (g_next++ << 8) | (uint64_t)o->kind;
     g_objs[h] = o;
     return h;
-}
-
-template <class T>
-T *get(uint64_t h, Kind k)
-{
-    std::lock_guard<std::mutex> l(g_mu);
-    auto it = g_objs.find(h);
-    if (it == g_objs.end() || it->second->kind != k) return nullptr;
-    return static_cast<T *>(it->second);
 }
 
 Obj *unreg(uint64_t h, Kind k)
@@ -269,6 +145,28 @@ int tlas_build_into(Context *cx, hipStream_t s, Tlas *t, const solhip_instance *
                       t->bvh);
 }
 
+// emitter table: counts and offsets on the device, one read-back of the
+// total to size the table
+int build_lights(hipStream_t s, const DevTables &tb, const Tlas *t, DevBuf &offsets, DevBuf &lights, uint32_t *nl_out,
+                 Timer *timer)
+{
+    const uint32_t n_inst = std::min(t->n_inst, tb.instance_count);
+    if (offsets.reserve(((uint64_t)n_inst + 1) * 4)) return SOLHIP_ERROR_OUT_OF_MEMORY;
+    uint32_t *offs = (uint32_t *)offsets.ptr;
+    int rc = launch_lights_count(s, tb, (const uint32_t *)t->inst_tris_dev.ptr, n_inst, offs, timer);
+    if (rc) return rc;
+    uint32_t nl = 0;
+    SOLHIP_HIP_CHECK(hipMemcpyAsync(&nl, offs + n_inst, 4, hipMemcpyDeviceToHost, s));
+    SOLHIP_HIP_CHECK(hipStreamSynchronize(s));
+    if (nl) {
+        if (lights.reserve((uint64_t)nl * sizeof(LightTri))) return SOLHIP_ERROR_OUT_OF_MEMORY;
+        rc = launch_lights_fill(s, tb, offs, n_inst, (LightTri *)lights.ptr, nl, timer);
+        if (rc) return rc;
+    }
+    *nl_out = nl;
+    return SOLHIP_OK;
+}
+
 // SOLHIP_PROG_PATHTRACE: buffers, emitter table, the wavefront passes
 int trace_wavefront(Context *c, hipStream_t s, Program *p, Tlas *t, const ProgArgs &a)
 {
@@ -326,21 +224,9 @@ int trace_wavefront(Context *c, hipStream_t s, Program *p, Tlas *t, const ProgAr
     int rc0 = launch_set_frame(s, (FrameConsts *)p->fc.ptr, fc);
     if (rc0) return rc0;
     if (w.nee) {
-        // emitter table: counts and offsets on the device, one read-back of
-        // the total to size the table
-        const uint32_t n_inst = std::min(t->n_inst, a.tb.instance_count);
-        if (p->light_offsets.reserve(((uint64_t)n_inst + 1) * 4)) return SOLHIP_ERROR_OUT_OF_MEMORY;
-        uint32_t *offs = (uint32_t *)p->light_offsets.ptr;
-        int rc = launch_lights_count(s, a.tb, (const uint32_t *)t->inst_tris_dev.ptr, n_inst, offs, timer);
-        if (rc) return rc;
         uint32_t nl = 0;
-        SOLHIP_HIP_CHECK(hipMemcpyAsync(&nl, offs + n_inst, 4, hipMemcpyDeviceToHost, s));
-        SOLHIP_HIP_CHECK(hipStreamSynchronize(s));
-        if (nl) {
-            if (p->lights.reserve((uint64_t)nl * sizeof(LightTri))) return SOLHIP_ERROR_OUT_OF_MEMORY;
-            rc = launch_lights_fill(s, a.tb, offs, n_inst, (LightTri *)p->lights.ptr, nl, timer);
-            if (rc) return rc;
-        }
+        const int rc = build_lights(s, a.tb, t, p->light_offsets, p->lights, &nl, timer);
+        if (rc) return rc;
         w.lights = (const LightTri *)p->lights.ptr;
         w.nlights = nl;
     }

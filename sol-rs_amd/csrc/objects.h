@@ -1,0 +1,160 @@
+// objects.h -- the library's handle objects (what the opaque uint64
+// handles of include/solhip.h refer to) and the registry helpers shared by
+// the C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "bvh.h"
+#include "common.h"
+#include "kernels.h"
+
+namespace solhip {
+
+enum class Kind { Context, Stream, Buffer, Blas, Tlas, Program, Server };
+
+struct Obj {
+    Kind kind;
+    uint64_t ctx;
+    virtual ~Obj() = default;
+};
+
+// hipEvent pairs around launches, resolved when statistics are read
+struct EventTimer : Timer {
+    struct Rec {
+        int kernel;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> pending;
+    std::vector<Rec> open;
+    void begin(hipStream_t s, int kernel) override
+    {
+        Rec r{kernel, nullptr, nullptr};
+        if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return;
+        (void)hipEventRecord(r.a, s);
+        open.push_back(r);
+    }
+    void end(hipStream_t s, int kernel) override
+    {
+        for (size_t i = open.size(); i-- > 0;) {
+            if (open[i].kernel != kernel) continue;
+            (void)hipEventRecord(open[i].b, s);
+            pending.push_back(open[i]);
+            open.erase(open.begin() + (long)i);
+            return;
+        }
+    }
+};
+
+struct Context : Obj {
+    int device = 0;
+    uint32_t ncu = 256;   // compute units (persistent grids)
+    hipStream_t stream = nullptr;
+    Counters *counters = nullptr;  // [SOLHIP_KERNEL_COUNT]
+    uint32_t profile = 0;
+    EventTimer timer;
+    uint64_t launches[SOLHIP_KERNEL_COUNT] = {};
+    double total_ms[SOLHIP_KERNEL_COUNT] = {};
+    Timer *timer_or_null() { return (profile & SOLHIP_PROFILE_TIME) ? &timer : nullptr; }
+    void resolve_events()
+    {
+        for (auto &r : timer.pending) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess && r.kernel >= 0 && r.kernel < SOLHIP_KERNEL_COUNT) {
+                launches[r.kernel] += 1;
+                total_ms[r.kernel] += ms;
+            }
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        timer.pending.clear();
+    }
+};
+
+struct Stream : Obj {
+    hipStream_t s = nullptr;
+};
+
+struct Buffer : Obj {
+    void *ptr = nullptr;
+    uint64_t bytes = 0;
+};
+
+struct Blas : Obj {
+    DeviceBvh bvh;
+    DevScratch scratch;
+};
+
+struct Tlas : Obj {
+    DeviceBvh bvh;
+    DevScratch scratch;
+    DevBuf inst_dev;   // solhip_instance[]
+    DevBuf refs_dev;   // BlasRef[]
+    std::vector<uint64_t> blas_handles;   // distinct BLAS referenced by the instances
+    uint32_t n_inst = 0;
+    DevBuf inst_tris_dev;                 // u32 triangle count of each instance's BLAS
+};
+
+struct Program : Obj {
+    uint32_t kind_id = 0;
+    solhip_spec spec{};
+    DevBuf queue, qcount;
+    // SOLHIP_PROG_PATHTRACE: path state, queues, emitter table, frame constants
+    DevBuf st0, st1, psum, q0, q1, hits, sq, scontrib, counts, heads, lights, light_offsets, fc;
+    // captured frame (wf_graph): valid for the WfArgs it was captured with
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    WfArgs gkey{};
+    bool gvalid = false;
+    uint64_t graph_captures = 0;
+    void drop_graph()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+        gvalid = false;
+    }
+    void release_all()
+    {
+        drop_graph();
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
+        cap_stream = nullptr;
+        for (DevBuf *b : {&queue, &qcount, &st0, &st1, &psum, &q0, &q1, &hits, &sq, &scontrib, &counts, &heads,
+                          &lights, &light_offsets, &fc})
+            b->release();
+    }
+};
+
+extern std::mutex g_mu;
+extern std::unordered_map<uint64_t, Obj *> g_objs;
+
+uint64_t reg(Obj *o);
+Obj *unreg(uint64_t h, Kind k);
+
+template <class T>
+T *get(uint64_t h, Kind k)
+{
+    std::lock_guard<std::mutex> l(g_mu);
+    auto it = g_objs.find(h);
+    if (it == g_objs.end() || it->second->kind != k) return nullptr;
+    return static_cast<T *>(it->second);
+}
+
+int bad_handle(const char *what);
+int bad_arg(const std::string &what);
+Context *ctx_of(solhip_ctx c);
+hipStream_t stream_of(Context *cx, solhip_stream s, bool &ok);
+uint32_t stack_depth_of(const Tlas *t);
+SceneView view_of(const Tlas *t);
+// NEE emitter table of a scene into p's buffers; *nl = number of lights
+int build_lights(hipStream_t s, const DevTables &tb, const Tlas *t, DevBuf &offsets, DevBuf &lights, uint32_t *nl,
+                 Timer *timer);
+
+}  // namespace solhip

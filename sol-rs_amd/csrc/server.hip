@@ -1,0 +1,707 @@
+// server.hip -- the render-request server: jobs of mixed scene, image size
+// and sample count traced by one wavefront.
+//
+// Device state is a pool of path slots (one per pixel of every running job),
+// two extension queues, a shadow queue and a retire list.  One step is
+//
+//   extend  closest hits of every queued ray, each against its job's scene
+//   shade   shade_vertex (the same code as the per-frame wavefront): next
+//           ray, shadow ray, or the path's sample ends (-> retire list)
+//   shadow  any-hit tests; unoccluded contributions join the path radiance
+//   retire  the ended sample joins the pixel sum; the pixel's next sample
+//           starts in the same slot (its RNG runs on, as in pathtrace.rgen's
+//           sample loop) or, after the last sample, the pixel is accumulated
+//           into the job's image and counted done
+//
+// so per pixel the arithmetic and random draws are those of the per-frame
+// wavefront, in the same order: a job's image is bit-identical to a
+// solhip_trace of the same program.  Admission is first-come first-served
+// into free slot ranges; the host polls per-job done counters between steps.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <string>
+
+#include "objects.h"
+#include "pathshade.h"
+#include "shading.h"
+#include "traverse.h"
+
+namespace solhip {
+
+namespace {
+
+constexpr uint32_t kSrvBlock = 64;
+constexpr uint32_t kMaxJobs = 256;   // job index lives in 8 bits of the slot meta word
+
+// per-job constants (device array, one entry per running-job index)
+struct JobDev {
+    SceneView sv;
+    DevTables tb;
+    Cam cam;
+    const LightTri *lights;
+    float4 *accum;
+    uint32_t *out_rgba8;
+    uint32_t nlights;
+    float total;
+    uint32_t W, H, frame, spp, max_depth, rr_start, enable_sky, nee;
+    float alpha, rmax;
+    uint32_t base;   // first slot
+    uint32_t pad;
+};
+
+struct SrvArgs {
+    const JobDev *jobs;
+    float4 *st0, *st1, *psum;
+    uint32_t *pmeta;      // job | sample << 8
+    QRay *q[2];
+    solhip_hit *hits;
+    QRay *sq;
+    float4 *scontrib;
+    uint32_t *retire;
+    uint32_t *counts;     // [0], [1] extension queues, [2] shadow queue, [3] retire list
+    uint32_t *heads;      // [0] extend, [1] shadow
+    uint32_t *done;       // finished pixels per job index
+    Counters *counters;
+};
+
+__device__ __forceinline__ uint32_t srv_wave_append(uint32_t *counter, bool pred)
+{
+    const uint64_t m = __ballot(pred);
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t base = 0;
+    if (m) {
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+        base = __shfl(base, leader);
+    }
+    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+// pathtrace.rgen:35 for this job's camera
+__device__ __forceinline__ float job_tmin(const JobDev &J)
+{
+    float o[4];
+    mat4_mul(J.cam.view_inverse, 0.0f, 0.0f, 0.0f, 1.0f, o);
+    return fmaxf(1.0f, sqrtf(o[0] * o[0] + o[1] * o[1] + o[2] * o[2])) * 1e-3f;
+}
+
+// a new sample of pixel (x, y): jittered camera ray from the running RNG
+__device__ __forceinline__ QRay start_sample(const JobDev &J, uint32_t slot, uint32_t x, uint32_t y, uint32_t &rng,
+                                             float4 &s0, float4 &s1)
+{
+    const float jx = next_rand(rng);
+    const float jy = next_rand(rng);
+    V3 o, d;
+    camera_ray(J.cam.view_inverse, J.cam.projection_inverse, x, y, J.W, J.H, jx, jy, o, d);
+    s0 = make_float4(1.0f, 1.0f, 1.0f, as_f(rng));
+    s1 = make_float4(0.0f, 0.0f, 0.0f, as_f(1u << 16));
+    QRay r;
+    r.ox = o.x;
+    r.oy = o.y;
+    r.oz = o.z;
+    r.path = slot;
+    r.dx = d.x;
+    r.dy = d.y;
+    r.dz = d.z;
+    r.tmax = J.rmax;
+    return r;
+}
+
+__global__ void k_srv_set_job(JobDev *jobs, uint32_t *done, uint32_t j, JobDev v)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        jobs[j] = v;
+        done[j] = 0;
+    }
+}
+
+// first sample of every pixel of job j, appended to queue cur (8x8 tiles)
+__global__ void k_srv_admit(SrvArgs A, uint32_t j, uint32_t cur, uint32_t nslots)
+{
+    const JobDev &J = A.jobs[j];
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t tiles_x = (J.W + 7) >> 3, tile = t >> 6, lane = t & 63u;
+    const uint32_t x = (tile % tiles_x) * 8 + (lane & 7u), y = (tile / tiles_x) * 8 + (lane >> 3);
+    const bool valid = t < nslots && x < J.W && y < J.H;
+    QRay r;
+    if (valid) {
+        const uint32_t slot = J.base + y * J.W + x;
+        uint32_t rng = tea(x + y * J.W, J.frame);   // pathtrace.rgen:47
+        float4 s0, s1;
+        r = start_sample(J, slot, x, y, rng, s0, s1);
+        A.st0[slot] = s0;
+        A.st1[slot] = s1;
+        A.psum[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        A.pmeta[slot] = j;
+    }
+    const uint32_t at = srv_wave_append(A.counts + cur, valid);
+    if (valid) A.q[cur][at] = r;
+}
+
+template <bool ANY>
+__device__ __forceinline__ void srv_trace(const SrvArgs &A, const QRay *q, uint32_t cnt, uint32_t *head, int kernel)
+{
+    extern __shared__ uint2 stack[];
+    const uint32_t lane = threadIdx.x & 63u;
+    TravCounts tc{0, 0, 0, false};
+    uint32_t rays = 0;
+    for (;;) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(head, 64u);
+        base = __shfl(base, 0);
+        if (base >= cnt) break;
+        const uint32_t i = base + lane;
+        if (i < cnt) {
+            const QRay r = q[i];
+            const JobDev &J = A.jobs[A.pmeta[r.path] & 0xFFu];
+            const Hit h = trace_ray<ANY, false>(J.sv, v3(r.ox, r.oy, r.oz), v3(r.dx, r.dy, r.dz), job_tmin(J), r.tmax,
+                                                0xFFu, stack + threadIdx.x, kSrvBlock, tc);
+            ++rays;
+            if (!ANY) {
+                solhip_hit out;
+                out.t = h.t;
+                out.u = h.u;
+                out.v = h.v;
+                out.inst = h.inst;
+                out.prim = h.prim;
+                A.hits[i] = out;
+            } else if (h.inst == SOLHIP_MISS) {
+                const float4 c = A.scontrib[i];
+                float4 s1 = A.st1[r.path];
+                s1.x = s1.x + c.x;
+                s1.y = s1.y + c.y;
+                s1.z = s1.z + c.z;
+                A.st1[r.path] = s1;
+            }
+        }
+    }
+    count_rays<false>(A.counters + kernel, rays, tc);
+}
+
+__global__ void __launch_bounds__(kSrvBlock) k_srv_extend(SrvArgs A, uint32_t cur)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // filled by this step's shade / retire; last read a step ago
+        A.counts[cur ^ 1u] = 0;
+        A.counts[2] = 0;
+        A.counts[3] = 0;
+    }
+    srv_trace<false>(A, A.q[cur], A.counts[cur], A.heads + 0, SOLHIP_KERNEL_SRV_EXTEND);
+}
+
+__global__ void __launch_bounds__(kSrvBlock) k_srv_shadow(SrvArgs A)
+{
+    srv_trace<true>(A, A.sq, A.counts[2], A.heads + 1, SOLHIP_KERNEL_SRV_SHADOW);
+}
+
+__global__ void __launch_bounds__(kSrvBlock) k_srv_shade(SrvArgs A, uint32_t cur)
+{
+    const uint32_t cnt = A.counts[cur];
+    const uint32_t lane = threadIdx.x & 63u, nthreads = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < cnt; base += nthreads) {
+        const uint32_t i = base + lane;
+        ShadeOut o;
+        bool ended = false;
+        uint32_t slot = 0;
+        if (i < cnt) {
+            const QRay r = A.q[cur][i];
+            slot = r.path;
+            const JobDev &J = A.jobs[A.pmeta[slot] & 0xFFu];
+            ShadeParams P;
+            P.tb = J.tb;
+            P.enable_sky = J.enable_sky;
+            P.max_depth = J.max_depth;
+            P.rr_start = J.rr_start;
+            P.lights = J.lights;
+            P.nlights = J.nlights;
+            P.total = J.total;
+            P.nee = J.nee && J.nlights > 0 && J.total > 0.0f;
+            P.rmax = J.rmax;
+            float4 s0 = A.st0[slot], s1 = A.st1[slot];
+            shade_vertex(P, slot, r, A.hits[i], s0, s1, o);
+            A.st0[slot] = s0;
+            A.st1[slot] = s1;
+            ended = !o.cont;
+        }
+        const uint32_t s_next = srv_wave_append(A.counts + (cur ^ 1u), o.cont);
+        if (o.cont) A.q[cur ^ 1u][s_next] = o.nr;
+        const uint32_t s_sh = srv_wave_append(A.counts + 2, o.shadow);
+        if (o.shadow) {
+            A.sq[s_sh] = o.sr;
+            A.scontrib[s_sh] = o.sc;
+        }
+        const uint32_t s_rt = srv_wave_append(A.counts + 3, ended);
+        if (ended) A.retire[s_rt] = slot;
+    }
+}
+
+__global__ void __launch_bounds__(kSrvBlock) k_srv_retire(SrvArgs A, uint32_t cur)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // this step's extend and shadow are complete
+        A.heads[0] = 0;
+        A.heads[1] = 0;
+    }
+    const uint32_t cnt = A.counts[3];
+    const uint32_t lane = threadIdx.x & 63u, nthreads = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < cnt; base += nthreads) {
+        const uint32_t i = base + lane;
+        bool again = false;
+        QRay r;
+        if (i < cnt) {
+            const uint32_t slot = A.retire[i];
+            const uint32_t meta = A.pmeta[slot], job = meta & 0xFFu;
+            const uint32_t sample = (meta >> 8) + 1u;
+            const JobDev &J = A.jobs[job];
+            const uint32_t p = slot - J.base, x = p % J.W, y = p / J.W;
+            const float4 s1 = A.st1[slot], ps = A.psum[slot];
+            const float pc0 = ps.x + s1.x, pc1 = ps.y + s1.y, pc2 = ps.z + s1.z;
+            if (sample < J.spp) {
+                A.psum[slot] = make_float4(pc0, pc1, pc2, 0.0f);
+                uint32_t rng = as_u(A.st0[slot].w);
+                float4 s0n, s1n;
+                r = start_sample(J, slot, x, y, rng, s0n, s1n);
+                A.st0[slot] = s0n;
+                A.st1[slot] = s1n;
+                A.pmeta[slot] = job | (sample << 8);
+                again = true;
+            } else {
+                // pathtrace.rgen:88-102
+                const float inv = 1.0f / (float)J.spp;
+                const float c0 = inv * pc0, c1 = inv * pc1, c2 = inv * pc2;
+                float4 *px = J.accum + p;
+                const float4 old = *px;
+                float m0 = mixf(old.x, c0, J.alpha), m1 = mixf(old.y, c1, J.alpha), m2 = mixf(old.z, c2, J.alpha);
+                const bool bad = isnan(m0) || isnan(m1) || isnan(m2) || isinf(m0) || isinf(m1) || isinf(m2);
+                if (bad) {
+                    m0 = old.x;
+                    m1 = old.y;
+                    m2 = old.z;
+                }
+                *px = make_float4(m0, m1, m2, 1.0f);
+                if (J.out_rgba8) {
+                    const float g = 1.0f / 2.2f;
+                    const uint32_t r8 = to_unorm8(powf(m0, g)), g8 = to_unorm8(powf(m1, g)),
+                                   b8 = to_unorm8(powf(m2, g));
+                    J.out_rgba8[p] = r8 | (g8 << 8) | (b8 << 16) | (255u << 24);
+                }
+                atomicAdd(A.done + job, 1u);
+            }
+        }
+        const uint32_t at = srv_wave_append(A.counts + (cur ^ 1u), again);
+        if (again) A.q[cur ^ 1u][at] = r;
+    }
+}
+
+int srv_status(const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(std::string(what) + " launch: " + hipGetErrorString(e));
+        return SOLHIP_ERROR_DEVICE;
+    }
+    return 0;
+}
+
+double now_ms(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+// bytes of device state per path slot: st0, st1, psum (48), meta (4), two
+// extension queues (64), hits (20), shadow queue + contribution (48), retire (4)
+constexpr uint64_t kSlotBytes = 48 + 4 + 64 + 20 + 48 + 4;
+
+struct SrvScene {
+    SceneView sv;
+    DevTables tb;
+    DevBuf light_offsets, lights;
+    uint32_t nlights = 0;
+    float total = 0.0f;
+};
+
+struct SrvJob {
+    solhip_job_desc desc{};
+    uint32_t state = SOLHIP_JOB_QUEUED;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    uint32_t base = 0, npix = 0, jidx = 0;
+};
+
+struct Server : Obj {
+    Context *cx = nullptr;
+    hipStream_t stream = nullptr;
+    uint32_t max_slots = 0, max_jobs = 0, max_stack = 1;
+    DevBuf jobs, st0, st1, psum, pmeta, q0, q1, hits, sq, scontrib, retire, counts, heads, done;
+    uint32_t *done_host = nullptr;   // pinned copy of the done counters
+    std::vector<SrvScene *> scenes;
+    std::map<uint64_t, SrvJob> all;
+    std::deque<uint64_t> waiting;
+    std::vector<uint64_t> running_by_idx;   // job id per job index (0 = free)
+    std::map<uint32_t, uint32_t> free_ranges;   // start -> length
+    uint64_t next_id = 1;
+    uint32_t cur = 0;
+    std::chrono::steady_clock::time_point t0;
+
+    SrvArgs args() const
+    {
+        SrvArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.jobs = (const JobDev *)jobs.ptr;
+        a.st0 = (float4 *)st0.ptr;
+        a.st1 = (float4 *)st1.ptr;
+        a.psum = (float4 *)psum.ptr;
+        a.pmeta = (uint32_t *)pmeta.ptr;
+        a.q[0] = (QRay *)q0.ptr;
+        a.q[1] = (QRay *)q1.ptr;
+        a.hits = (solhip_hit *)hits.ptr;
+        a.sq = (QRay *)sq.ptr;
+        a.scontrib = (float4 *)scontrib.ptr;
+        a.retire = (uint32_t *)retire.ptr;
+        a.counts = (uint32_t *)counts.ptr;
+        a.heads = (uint32_t *)heads.ptr;
+        a.done = (uint32_t *)done.ptr;
+        a.counters = cx->counters;
+        return a;
+    }
+    bool alloc_range(uint32_t n, uint32_t &base)
+    {
+        for (auto it = free_ranges.begin(); it != free_ranges.end(); ++it) {
+            if (it->second < n) continue;
+            base = it->first;
+            const uint32_t rest = it->second - n;
+            free_ranges.erase(it);
+            if (rest) free_ranges[base + n] = rest;
+            return true;
+        }
+        return false;
+    }
+    void free_range(uint32_t base, uint32_t n)
+    {
+        auto it = free_ranges.emplace(base, n).first;
+        auto nx = std::next(it);
+        if (nx != free_ranges.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_ranges.erase(nx);
+        }
+        if (it != free_ranges.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_ranges.erase(it);
+            }
+        }
+    }
+    void release_all()
+    {
+        for (DevBuf *b : {&jobs, &st0, &st1, &psum, &pmeta, &q0, &q1, &hits, &sq, &scontrib, &retire, &counts, &heads,
+                          &done})
+            b->release();
+        for (SrvScene *sc : scenes) {
+            sc->light_offsets.release();
+            sc->lights.release();
+            delete sc;
+        }
+        scenes.clear();
+        if (done_host) (void)hipHostFree(done_host);
+        done_host = nullptr;
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+
+    // admit waiting jobs (first come, first served) while slots and job
+    // indices are free
+    int admit()
+    {
+        while (!waiting.empty()) {
+            SrvJob &jb = all[waiting.front()];
+            uint32_t jidx = 0;
+            while (jidx < max_jobs && running_by_idx[jidx]) ++jidx;
+            if (jidx == max_jobs) return 0;
+            uint32_t base = 0;
+            if (!alloc_range(jb.npix, base)) return 0;
+            const SrvScene *sc = scenes[jb.desc.scene];
+            Buffer *acc = get<Buffer>(jb.desc.accum, Kind::Buffer);
+            Buffer *out = jb.desc.out_rgba8 ? get<Buffer>(jb.desc.out_rgba8, Kind::Buffer) : nullptr;
+            if (!acc) {
+                free_range(base, jb.npix);
+                return bad_handle("job accum buffer");
+            }
+            JobDev v;
+            std::memset(&v, 0, sizeof(v));
+            v.sv = sc->sv;
+            v.tb = sc->tb;
+            std::memcpy(v.cam.view_inverse, jb.desc.uniforms.view_inverse, 64);
+            std::memcpy(v.cam.projection_inverse, jb.desc.uniforms.projection_inverse, 64);
+            v.lights = (const LightTri *)sc->lights.ptr;
+            v.nlights = sc->nlights;
+            v.total = sc->total;
+            v.accum = (float4 *)acc->ptr;
+            v.out_rgba8 = out ? (uint32_t *)out->ptr : nullptr;
+            v.W = jb.desc.W;
+            v.H = jb.desc.H;
+            v.frame = jb.desc.uniforms.frame[2];
+            v.spp = jb.desc.spec.spp_per_frame;
+            v.max_depth = jb.desc.spec.max_depth;
+            v.rr_start = jb.desc.spec.rr_start;
+            v.enable_sky = jb.desc.spec.enable_sky;
+            v.nee = jb.desc.spec.nee;
+            const uint32_t denom = jb.desc.uniforms.frame[2] - (uint32_t)jb.desc.accum_start_frame + 1u;
+            v.alpha = 1.0f / (float)denom;   // pathtrace.rgen:90
+            v.rmax = 10000.0f;
+            v.base = base;
+            hipLaunchKernelGGL(k_srv_set_job, dim3(1), dim3(64), 0, stream, (JobDev *)jobs.ptr, (uint32_t *)done.ptr,
+                               jidx, v);
+            const uint32_t tiles = ((v.W + 7) / 8) * ((v.H + 7) / 8), nslots = tiles * 64;
+            Timer *timer = cx->timer_or_null();
+            if (timer) timer->begin(stream, SOLHIP_KERNEL_SRV_ADMIT);
+            hipLaunchKernelGGL(k_srv_admit, dim3((nslots + kSrvBlock - 1) / kSrvBlock), dim3(kSrvBlock), 0, stream,
+                               args(), jidx, cur, nslots);
+            if (timer) timer->end(stream, SOLHIP_KERNEL_SRV_ADMIT);
+            int rc = srv_status("server admit");
+            if (rc) return rc;
+            jb.state = SOLHIP_JOB_RUNNING;
+            jb.base = base;
+            jb.jidx = jidx;
+            jb.t_admit = now_ms(t0);
+            running_by_idx[jidx] = waiting.front();
+            waiting.pop_front();
+        }
+        return 0;
+    }
+
+    int step()
+    {
+        const SrvArgs a = args();
+        const uint32_t persist = cx->ncu * 16;
+        const uint32_t shade_blocks = std::min<uint32_t>((max_slots + kSrvBlock - 1) / kSrvBlock, cx->ncu * 32);
+        const size_t lds = (size_t)max_stack * sizeof(uint2) * kSrvBlock;
+        Timer *timer = cx->timer_or_null();
+        auto timed = [&](int id, auto &&launch) {
+            if (timer) timer->begin(stream, id);
+            launch();
+            if (timer) timer->end(stream, id);
+        };
+        timed(SOLHIP_KERNEL_SRV_EXTEND,
+              [&] { hipLaunchKernelGGL(k_srv_extend, dim3(persist), dim3(kSrvBlock), lds, stream, a, cur); });
+        timed(SOLHIP_KERNEL_SRV_SHADE,
+              [&] { hipLaunchKernelGGL(k_srv_shade, dim3(shade_blocks), dim3(kSrvBlock), 0, stream, a, cur); });
+        timed(SOLHIP_KERNEL_SRV_SHADOW,
+              [&] { hipLaunchKernelGGL(k_srv_shadow, dim3(persist), dim3(kSrvBlock), lds, stream, a); });
+        timed(SOLHIP_KERNEL_SRV_RETIRE,
+              [&] { hipLaunchKernelGGL(k_srv_retire, dim3(shade_blocks), dim3(kSrvBlock), 0, stream, a, cur); });
+        cur ^= 1u;
+        return srv_status("server step");
+    }
+
+    // reads the done counters; finished jobs give back their slots
+    int poll()
+    {
+        SOLHIP_HIP_CHECK(hipMemcpyAsync(done_host, done.ptr, sizeof(uint32_t) * max_jobs, hipMemcpyDeviceToHost,
+                                        stream));
+        SOLHIP_HIP_CHECK(hipStreamSynchronize(stream));
+        const double t = now_ms(t0);
+        for (uint32_t j = 0; j < max_jobs; ++j) {
+            const uint64_t id = running_by_idx[j];
+            if (!id) continue;
+            SrvJob &jb = all[id];
+            if (done_host[j] >= jb.npix) {
+                jb.state = SOLHIP_JOB_DONE;
+                jb.t_done = t;
+                free_range(jb.base, jb.npix);
+                running_by_idx[j] = 0;
+            }
+        }
+        return 0;
+    }
+    uint32_t live() const
+    {
+        uint32_t n = (uint32_t)waiting.size();
+        for (uint64_t id : running_by_idx) n += id ? 1u : 0u;
+        return n;
+    }
+};
+
+}  // namespace solhip
+
+using namespace solhip;
+
+extern "C" {
+
+int32_t solhip_server_create(solhip_ctx ctx, uint64_t hbm_budget, uint32_t max_jobs, solhip_server *out)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    if (!out) return bad_arg("out");
+    if (max_jobs == 0 || max_jobs > kMaxJobs) return bad_arg("max_jobs must be in [1, 256]");
+    const uint64_t slots = hbm_budget / kSlotBytes;
+    if (slots < 64 || slots > 0xFFFFFF00ull) return bad_arg("HBM budget out of range");
+    Server *sv = new Server();
+    sv->kind = Kind::Server;
+    sv->ctx = ctx;
+    sv->cx = c;
+    sv->max_slots = (uint32_t)slots;
+    sv->max_jobs = max_jobs;
+    sv->running_by_idx.assign(max_jobs, 0);
+    sv->free_ranges[0] = sv->max_slots;
+    sv->t0 = std::chrono::steady_clock::now();
+    const uint64_t n = slots;
+    bool fail = sv->jobs.reserve((uint64_t)max_jobs * sizeof(JobDev)) || sv->st0.reserve(n * 16) ||
+                sv->st1.reserve(n * 16) || sv->psum.reserve(n * 16) || sv->pmeta.reserve(n * 4) ||
+                sv->q0.reserve(n * sizeof(QRay)) || sv->q1.reserve(n * sizeof(QRay)) ||
+                sv->hits.reserve(n * sizeof(solhip_hit)) || sv->sq.reserve(n * sizeof(QRay)) ||
+                sv->scontrib.reserve(n * 16) || sv->retire.reserve(n * 4) || sv->counts.reserve(256) ||
+                sv->heads.reserve(256) || sv->done.reserve((uint64_t)max_jobs * 4);
+    if (!fail) fail = hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking) != hipSuccess ||
+                      hipHostMalloc((void **)&sv->done_host, sizeof(uint32_t) * max_jobs, hipHostMallocDefault) !=
+                          hipSuccess;
+    if (fail) {
+        sv->release_all();
+        delete sv;
+        set_error("render server allocation failed");
+        return SOLHIP_ERROR_OUT_OF_MEMORY;
+    }
+    if (launch_fill(sv->stream, (uint32_t *)sv->counts.ptr, 64, 0u) ||
+        launch_fill(sv->stream, (uint32_t *)sv->heads.ptr, 64, 0u) ||
+        launch_fill(sv->stream, (uint32_t *)sv->done.ptr, max_jobs, 0u) || hipStreamSynchronize(sv->stream)) {
+        sv->release_all();
+        delete sv;
+        return SOLHIP_ERROR_DEVICE;
+    }
+    *out = reg(sv);
+    return SOLHIP_OK;
+}
+
+int32_t solhip_server_destroy(solhip_ctx ctx, solhip_server h)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    Server *sv = static_cast<Server *>(unreg(h, Kind::Server));
+    if (!sv) return bad_handle("server");
+    (void)hipStreamSynchronize(sv->stream);
+    sv->release_all();
+    delete sv;
+    return SOLHIP_OK;
+}
+
+int32_t solhip_server_scene(solhip_ctx ctx, solhip_server h, solhip_tlas th, const solhip_scene_tables *tables,
+                            uint32_t *scene_id)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    Server *sv = get<Server>(h, Kind::Server);
+    if (!sv) return bad_handle("server");
+    Tlas *t = get<Tlas>(th, Kind::Tlas);
+    if (!t) return bad_handle("TLAS");
+    if (!tables || !tables->instances_addr || !tables->vertex_table_addr || !tables->index_table_addr ||
+        !tables->material_table_addr || (tables->index_size != 4 && tables->index_size != 8) || !scene_id)
+        return bad_arg("scene tables");
+    if (stack_depth_of(t) > (uint32_t)kMaxStack) {
+        set_error("acceleration structures too deep for the traversal stack");
+        return SOLHIP_ERROR_UNSUPPORTED;
+    }
+    SrvScene *sc = new SrvScene();
+    sc->sv = view_of(t);
+    sc->tb.instances = (const solhip_scene_instance *)tables->instances_addr;
+    sc->tb.vertex_table = (const uint64_t *)tables->vertex_table_addr;
+    sc->tb.index_table = (const uint64_t *)tables->index_table_addr;
+    sc->tb.material_table = (const uint64_t *)tables->material_table_addr;
+    sc->tb.instance_count = tables->instance_count;
+    sc->tb.index_size = tables->index_size;
+    uint32_t nl = 0;
+    int rc = build_lights(sv->stream, sc->tb, t, sc->light_offsets, sc->lights, &nl, nullptr);
+    if (rc) {
+        sc->light_offsets.release();
+        sc->lights.release();
+        delete sc;
+        return rc;
+    }
+    sc->nlights = nl;
+    if (nl) {
+        LightTri last;
+        SOLHIP_HIP_CHECK(hipMemcpyAsync(&last, (const LightTri *)sc->lights.ptr + (nl - 1), sizeof(LightTri),
+                                        hipMemcpyDeviceToHost, sv->stream));
+        SOLHIP_HIP_CHECK(hipStreamSynchronize(sv->stream));
+        sc->total = last.cdf;
+    }
+    sv->max_stack = std::max(sv->max_stack, sc->sv.stack_depth);
+    sv->scenes.push_back(sc);
+    *scene_id = (uint32_t)(sv->scenes.size() - 1);
+    return SOLHIP_OK;
+}
+
+int32_t solhip_server_submit(solhip_ctx ctx, solhip_server h, const solhip_job_desc *job, uint64_t *job_id)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    Server *sv = get<Server>(h, Kind::Server);
+    if (!sv) return bad_handle("server");
+    if (!job || !job_id) return bad_arg("job");
+    if (job->scene >= sv->scenes.size()) return bad_arg("scene id");
+    if (job->W == 0 || job->H == 0) return bad_arg("image size");
+    const uint64_t npix = (uint64_t)job->W * job->H;
+    if (npix > sv->max_slots) return bad_arg("image larger than the server's slot budget");
+    if (job->spec.spp_per_frame == 0 || job->spec.spp_per_frame > 0xFFFFFFu || job->spec.max_depth > 4096 ||
+        job->spec.enable_sky > 2)
+        return bad_arg("job spec");
+    Buffer *acc = get<Buffer>(job->accum, Kind::Buffer);
+    if (!acc || acc->bytes < npix * 16) return bad_arg("job accum buffer");
+    if (job->out_rgba8) {
+        Buffer *o = get<Buffer>(job->out_rgba8, Kind::Buffer);
+        if (!o || o->bytes < npix * 4) return bad_arg("job out_rgba8 buffer");
+    }
+    SrvJob jb;
+    jb.desc = *job;
+    jb.npix = (uint32_t)npix;
+    jb.t_submit = now_ms(sv->t0);
+    const uint64_t id = sv->next_id++;
+    sv->all[id] = jb;
+    sv->waiting.push_back(id);
+    *job_id = id;
+    return SOLHIP_OK;
+}
+
+int32_t solhip_server_run(solhip_ctx ctx, solhip_server h, uint32_t max_steps, uint32_t *live)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    Server *sv = get<Server>(h, Kind::Server);
+    if (!sv) return bad_handle("server");
+    for (uint32_t k = 0; k < max_steps; ++k) {
+        int rc = sv->admit();
+        if (rc) return rc;
+        bool running = false;
+        for (uint64_t id : sv->running_by_idx) running = running || id != 0;
+        if (!running) break;
+        rc = sv->step();
+        if (rc) return rc;
+        rc = sv->poll();
+        if (rc) return rc;
+    }
+    if (live) *live = sv->live();
+    return SOLHIP_OK;
+}
+
+int32_t solhip_server_job(solhip_ctx ctx, solhip_server h, uint64_t job_id, solhip_job_status *out)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return bad_handle("context");
+    Server *sv = get<Server>(h, Kind::Server);
+    if (!sv) return bad_handle("server");
+    auto it = sv->all.find(job_id);
+    if (it == sv->all.end() || !out) return bad_arg("job id");
+    std::memset(out, 0, sizeof(*out));
+    out->state = it->second.state;
+    out->submit_ms = it->second.t_submit;
+    out->admit_ms = it->second.t_admit;
+    out->done_ms = it->second.t_done;
+    return SOLHIP_OK;
+}
+
+}  // extern "C"

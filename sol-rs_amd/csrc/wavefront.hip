@@ -26,14 +26,13 @@
 #include <string>
 
 #include "kernels.h"
+#include "pathshade.h"
 #include "shading.h"
 #include "traverse.h"
 
 namespace solhip {
 
 namespace {
-
-constexpr float kPi = 0x1.921fb6p+1f;
 
 // slot of the first queue -> pixel
 __device__ __forceinline__ bool slot_pixel(uint32_t slot, uint32_t W, uint32_t H, uint32_t &x, uint32_t &y)
@@ -64,23 +63,6 @@ __device__ __forceinline__ float ray_tmin(const WfArgs &A)
     float o[4];
     mat4_mul(A.fc->cam.view_inverse, 0.0f, 0.0f, 0.0f, 1.0f, o);
     return fmaxf(1.0f, sqrtf(o[0] * o[0] + o[1] * o[1] + o[2] * o[2])) * 1e-3f;
-}
-
-__device__ __forceinline__ bool is_emitter(const float *mat)
-{
-    return mat[4] >= 1.0f || mat[5] >= 1.0f || mat[6] >= 1.0f;   // pathtrace.rchit:71
-}
-
-// first light whose inclusive area sum exceeds u (the last one if none)
-__device__ __forceinline__ uint32_t pick_light(const LightTri *L, uint32_t n, float u)
-{
-    uint32_t lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (L[mid].cdf > u) hi = mid;
-        else lo = mid + 1;
-    }
-    return lo;
 }
 
 __global__ void k_pt_generate(WfArgs A, uint32_t pass)
@@ -203,137 +185,35 @@ __global__ void k_pt_shade(WfArgs A, uint32_t b)
     QRay *qout = A.q[(b + 1) & 1];
     uint32_t *cout = A.counts + ((b + 1) & 1);
     const uint32_t lane = threadIdx.x & 63u, nthreads = gridDim.x * blockDim.x;
-    const bool sun = A.enable_sky != 2;
-    const float total = A.nlights ? A.lights[A.nlights - 1].cdf : 0.0f;
-    const bool nee = A.nee && A.nlights > 0 && total > 0.0f;
+    ShadeParams P;
+    P.tb = A.tb;
+    P.enable_sky = A.enable_sky;
+    P.max_depth = A.max_depth;
+    P.rr_start = A.rr_start;
+    P.lights = A.lights;
+    P.nlights = A.nlights;
+    P.total = A.nlights ? A.lights[A.nlights - 1].cdf : 0.0f;
+    P.nee = A.nee && A.nlights > 0 && P.total > 0.0f;
+    P.rmax = A.rmax;
     for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < cnt; base += nthreads) {
         const uint32_t i = base + lane;
-        bool cont = false, shadow = false;
-        QRay nr, sr;
-        float4 sc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        ShadeOut o;
         if (i < cnt) {
             const QRay r = qin[i];
             if (r.path != kNoPath) {
                 const uint32_t p = r.path;
-                const solhip_hit h = A.hits[i];
-                const float4 s0 = A.st0[p], s1 = A.st1[p];
-                V3 T = v3(s0.x, s0.y, s0.z);
-                uint32_t rng = as_u(s0.w);
-                V3 L = v3(s1.x, s1.y, s1.z);
-                const uint32_t fl = as_u(s1.w);
-                uint32_t depth = fl & 0xFFFFu;
-                bool count_emit = (fl >> 16) & 1u;
-                const V3 d = v3(r.dx, r.dy, r.dz);
-                if (h.inst == SOLHIP_MISS) {
-                    const V3 sky = pt_miss(A.enable_sky, sun, d);
-                    L = v3(L.x + T.x * sky.x, L.y + T.y * sky.y, L.z + T.z * sky.z);
-                } else {
-                    const float *mat = reinterpret_cast<const float *>(A.tb.material_table[h.inst]);
-                    if (is_emitter(mat)) {
-                        if (count_emit) L = v3(L.x + T.x * mat[4], L.y + T.y * mat[5], L.z + T.z * mat[6]);
-                    } else {
-                        const Surf sf = fetch_surface(A.tb, h.inst, h.prim, h.u, h.v);
-                        const V3 wI = normalize(d);
-                        const float sg = sgn(dot(sf.nrm, neg(wI)));
-                        const V3 nO = scale(sf.nrm, sg);
-                        const float metallic = mat[8], roughness = mat[9];
-                        const float alphaSquared = roughness * roughness;
-                        const float xi0 = next_rand(rng);
-                        const float xi1 = next_rand(rng);
-                        const float rnd = next_rand(rng);
-                        const V3 o = v3(sf.pos.x + 0.0001f * nO.x, sf.pos.y + 0.0001f * nO.y,
-                                        sf.pos.z + 0.0001f * nO.z);
-                        const V3 base_c = v3(mat[0] * sf.col.x, mat[1] * sf.col.y, mat[2] * sf.col.z);
-                        V3 nd, hv;
-                        bool diffuse = false;
-                        if (rnd < metallic) {
-                            nd = sample_ggx(reflect(d, nO), xi0, xi1, alphaSquared);
-                            hv = base_c;
-                        } else {
-                            const V3 m = sample_ggx(nO, xi0, xi1, alphaSquared);
-                            if (rnd < fresnel_dielectric(nO, m, 1.0f / 1.5f)) {
-                                nd = reflect(d, m);
-                                hv = v3(1.0f, 1.0f, 1.0f);
-                            } else {
-                                nd = sample_cosine(nO, xi0, xi1);
-                                hv = base_c;
-                                diffuse = true;
-                            }
-                        }
-                        // throughput into this vertex times its diffuse albedo
-                        const V3 Tb = v3(T.x * base_c.x, T.y * base_c.y, T.z * base_c.z);
-                        T = v3(T.x * hv.x, T.y * hv.y, T.z * hv.z);
-                        depth++;
-                        if (depth <= A.max_depth) {
-                            count_emit = true;
-                            if (nee) {
-                                // the diffuse lobe's share of direct light: pathtrace.rchit:93-110
-                                // takes it when rand >= max(metallic, F(m)), m drawn from the same
-                                // Xi as the cosine direction (same azimuth, cos^2 theta_m =
-                                // (1 - c^2) / (c^2 (a^2 - 1) + 1) for a direction at cos theta = c)
-                                const float ul = next_rand(rng) * total;
-                                const float b1 = next_rand(rng), b2 = next_rand(rng);
-                                const LightTri &lt = A.lights[pick_light(A.lights, A.nlights, ul)];
-                                const float su = sqrtf(b1), ba = 1.0f - su, bb = su * (1.0f - b2), bc = su * b2;
-                                const V3 dl = v3((lt.p0[0] * ba + lt.p1[0] * bb + lt.p2[0] * bc) - o.x,
-                                                 (lt.p0[1] * ba + lt.p1[1] * bb + lt.p2[1] * bc) - o.y,
-                                                 (lt.p0[2] * ba + lt.p1[2] * bb + lt.p2[2] * bc) - o.z);
-                                const float dist2 = dot(dl, dl), dist = sqrtf(dist2);
-                                const V3 wl = v3(dl.x / dist, dl.y / dist, dl.z / dist);
-                                const float cos_x = dot(nO, wl);
-                                const float cos_l = fabsf(dot(v3(lt.n[0], lt.n[1], lt.n[2]), wl));
-                                if (dist2 > 0.0f && cos_x > 0.0f && cos_l > 0.0f) {
-                                    const float x0 = cos_x * cos_x;
-                                    const float cm =
-                                        sqrtf(saturate((1.0f - x0) / (x0 * (alphaSquared - 1.0f) + 1.0f)));
-                                    const float pd = 1.0f - fmaxf(metallic, fresnel_cos(cm, 1.0f / 1.5f));
-                                    if (pd > 0.0f) {
-                                        const float g = (pd * (cos_x * cos_l)) * total / (kPi * dist2);
-                                        sc = make_float4((Tb.x * lt.e[0]) * g, (Tb.y * lt.e[1]) * g,
-                                                         (Tb.z * lt.e[2]) * g, 0.0f);
-                                        sr.ox = o.x;
-                                        sr.oy = o.y;
-                                        sr.oz = o.z;
-                                        sr.path = p;
-                                        sr.dx = wl.x;
-                                        sr.dy = wl.y;
-                                        sr.dz = wl.z;
-                                        sr.tmax = dist * 0.9999f;
-                                        shadow = true;
-                                    }
-                                }
-                                // emitters met by the diffuse lobe were sampled above
-                                if (diffuse) count_emit = false;
-                            }
-                            cont = true;
-                            if (A.rr_start && depth >= A.rr_start) {
-                                const float q = fminf(fmaxf(T.x, fmaxf(T.y, T.z)), 0.95f);
-                                if (next_rand(rng) >= q) cont = false;
-                                else T = v3(T.x / q, T.y / q, T.z / q);
-                            }
-                            if (cont) {
-                                nr.ox = o.x;
-                                nr.oy = o.y;
-                                nr.oz = o.z;
-                                nr.path = p;
-                                nr.dx = nd.x;
-                                nr.dy = nd.y;
-                                nr.dz = nd.z;
-                                nr.tmax = A.rmax;
-                            }
-                        }
-                    }
-                }
-                A.st0[p] = make_float4(T.x, T.y, T.z, as_f(rng));
-                A.st1[p] = make_float4(L.x, L.y, L.z, as_f((depth & 0xFFFFu) | (count_emit ? 1u << 16 : 0u)));
+                float4 s0 = A.st0[p], s1 = A.st1[p];
+                shade_vertex(P, p, r, A.hits[i], s0, s1, o);
+                A.st0[p] = s0;
+                A.st1[p] = s1;
             }
         }
-        const uint32_t s_next = wave_append(cout, cont);
-        if (cont) qout[s_next] = nr;
-        const uint32_t s_sh = wave_append(A.counts + 2, shadow);
-        if (shadow) {
-            A.sq[s_sh] = sr;
-            A.scontrib[s_sh] = sc;
+        const uint32_t s_next = wave_append(cout, o.cont);
+        if (o.cont) qout[s_next] = o.nr;
+        const uint32_t s_sh = wave_append(A.counts + 2, o.shadow);
+        if (o.shadow) {
+            A.sq[s_sh] = o.sr;
+            A.scontrib[s_sh] = o.sc;
         }
     }
 }

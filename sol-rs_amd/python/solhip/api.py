@@ -61,8 +61,22 @@ class KernelStats(C.Structure):
     _fields_ = [("launches", C.c_uint64), ("total_ms", C.c_double), ("counters", Counters)]
 
 
+class JobDesc(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("W", C.c_uint32), ("H", C.c_uint32), ("spec", Spec),
+                ("uniforms", C.c_uint8 * 400), ("accum_start_frame", C.c_int32), ("reserved", C.c_uint32),
+                ("accum", C.c_uint64), ("out_rgba8", C.c_uint64)]
+
+
+class JobStatus(C.Structure):
+    _fields_ = [("state", C.c_uint32), ("reserved", C.c_uint32), ("submit_ms", C.c_double),
+                ("admit_ms", C.c_double), ("done_ms", C.c_double)]
+
+
 KERNELS = {"trace_rays": 0, "ao_primary": 1, "ao_rays": 2, "ao_glsl": 3, "pathtrace_glsl": 4, "debug": 5,
-           "pt_generate": 6, "pt_extend": 7, "pt_shade": 8, "pt_shadow": 9, "pt_finalize": 10, "pt_lights": 11}
+           "pt_generate": 6, "pt_extend": 7, "pt_shade": 8, "pt_shadow": 9, "pt_finalize": 10, "pt_lights": 11,
+           "srv_admit": 12, "srv_extend": 13, "srv_shade": 14, "srv_shadow": 15, "srv_retire": 16}
+
+JOB_QUEUED, JOB_RUNNING, JOB_DONE = 0, 1, 2
 PROFILE_TIME, PROFILE_COUNT = 0x1, 0x2
 
 
@@ -106,6 +120,12 @@ SIGNATURES = {
     "solhip_pathtrace_autotune": (_i32, [_u64, _u64, _u64, C.POINTER(SceneTables), _vp, C.POINTER(Images), _u32, _u32,
                                          C.POINTER(Spec), _u32, C.c_char_p, C.POINTER(Spec),
                                          C.POINTER(C.c_float)]),
+    "solhip_server_create": (_i32, [_u64, _u64, _u32, C.POINTER(_u64)]),
+    "solhip_server_destroy": (_i32, [_u64, _u64]),
+    "solhip_server_scene": (_i32, [_u64, _u64, _u64, C.POINTER(SceneTables), C.POINTER(_u32)]),
+    "solhip_server_submit": (_i32, [_u64, _u64, C.POINTER(JobDesc), C.POINTER(_u64)]),
+    "solhip_server_run": (_i32, [_u64, _u64, _u32, C.POINTER(_u32)]),
+    "solhip_server_job": (_i32, [_u64, _u64, _u64, C.POINTER(JobStatus)]),
 }
 
 _lib = None
@@ -342,3 +362,47 @@ class GpuScene:
         descs["blas"] = [self.blas[int(k)] for k in instance_descs["blas"]]
         self.descs = descs
         self.ctx.tlas_rebuild(self.tlas, descs)
+
+
+class Server:
+    """render-request server: resident scenes, jobs of mixed scene / size /
+    samples traced by one wavefront (solhip_server_*)"""
+
+    def __init__(self, ctx, hbm_budget, max_jobs=64):
+        self.ctx = ctx
+        h = C.c_uint64()
+        check(lib().solhip_server_create(ctx.h, hbm_budget, max_jobs, C.byref(h)))
+        self.h = h.value
+
+    def scene(self, gpu_scene):
+        sid = C.c_uint32()
+        check(lib().solhip_server_scene(self.ctx.h, self.h, gpu_scene.tlas, C.byref(gpu_scene.tables), C.byref(sid)))
+        return sid.value
+
+    def submit(self, scene, W, H, spec, uniforms, accum, out_rgba8=None, accum_start=0):
+        d = JobDesc()
+        d.scene, d.W, d.H, d.spec = scene, W, H, spec
+        u = np.ascontiguousarray(uniforms, UNIFORMS).view(np.uint8).reshape(-1)
+        C.memmove(d.uniforms, u.ctypes.data, 400)
+        d.accum_start_frame = accum_start
+        d.accum = accum.h
+        d.out_rgba8 = out_rgba8.h if out_rgba8 is not None else 0
+        jid = C.c_uint64()
+        check(lib().solhip_server_submit(self.ctx.h, self.h, C.byref(d), C.byref(jid)))
+        return jid.value
+
+    def run(self, max_steps=1 << 30):
+        live = C.c_uint32()
+        check(lib().solhip_server_run(self.ctx.h, self.h, max_steps, C.byref(live)))
+        return live.value
+
+    def job(self, jid):
+        st = JobStatus()
+        check(lib().solhip_server_job(self.ctx.h, self.h, jid, C.byref(st)))
+        return st
+
+    def close(self):
+        if self.h:
+            check(lib().solhip_server_destroy(self.ctx.h, self.h))
+            self.h = 0
+
