@@ -63,7 +63,8 @@ def parse_args():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", type=int, default=0)
-    ap.add_argument("--workload", default="ao", choices=["ao", "pathtrace-c0", "pathtrace-c2", "server-c3"])
+    ap.add_argument("--workload", default="ao",
+                    choices=["ao", "pathtrace-c0", "pathtrace-c2", "server-c3", "cache-c4"])
     ap.add_argument("--jobs", type=int, default=40, help="server-c3: requests")
     ap.add_argument("--rate", type=float, default=200.0, help="server-c3: requests per second")
     ap.add_argument("--any-hit", action="store_true",
@@ -113,6 +114,9 @@ def main():
         return
     if a.workload == "server-c3":
         bench_server(a)
+        return
+    if a.workload == "cache-c4":
+        bench_cache(a)
         return
     W, H = (int(v) for v in a.res.lower().split("x"))
     import solhip
@@ -322,6 +326,72 @@ def bench_server(a):
                                "heightfield / 10k instances, resolutions 256^2..1080p, spp log-uniform 1-256, "
                                "depth 4-8, NEE + roulette" % (a.rate, a.jobs), "parallelism": "single GPU"},
         "extra": stats,
+    }
+    ctx.close()
+    print(json.dumps(result))
+
+
+def bench_cache(a):
+    """learned radiance cache against the plain tracer on the C2 scene:
+    relative RMSE against a 4096-spp plain reference as a function of wall
+    time, and the time the cache needs to reach the plain tracer's 1024-spp
+    error (if it does); one JSON line"""
+    import solhip
+    from solhip import scenes
+    W, H = (int(v) for v in a.res.lower().split("x"))
+    sd = scenes.instanced(10000)
+    ctx = solhip.Context(0)
+    gs = solhip.GpuScene(ctx, sd)
+
+    def spec(nrc, spp):
+        s = solhip.spec_default(solhip.PROG_PATHTRACE)
+        s.enable_sky, s.spp_per_frame, s.max_depth, s.rr_start, s.nee, s.nrc = 2, spp, 8, 3, 1, nrc
+        return s
+
+    acc = ctx.upload(np.zeros((H, W, 4), np.float32))
+    img = solhip.Images(acc.h, 0, 0, 0, 0, 0)
+    # reference: 4096 spp of the plain tracer (frames of 64 spp, running mean)
+    pr = ctx.program(solhip.PROG_PATHTRACE, spec(0, 64))
+    for f in range(64):
+        ctx.trace(pr, gs.tlas, gs.tables, sd.uniforms(W, H, 100000 + f), 100000, img, W, H)
+    ref = acc.download(np.float32).reshape(H, W, 4)[..., :3].astype(np.float64)
+    ctx.program_destroy(pr)
+    norm = float(np.sqrt((ref ** 2).mean()))
+
+    def run(nrc, frames, checkpoints):
+        p = ctx.program(solhip.PROG_PATHTRACE, spec(nrc, 1))
+        out = []
+        t = 0.0
+        for f in range(frames):
+            ctx.sync()
+            t0 = time.perf_counter()
+            ctx.trace(p, gs.tlas, gs.tables, sd.uniforms(W, H, f), 0, img, W, H)
+            ctx.sync()
+            t += time.perf_counter() - t0
+            if f + 1 in checkpoints:
+                im = acc.download(np.float32).reshape(H, W, 4)[..., :3].astype(np.float64)
+                out.append({"spp": f + 1, "seconds": round(t, 4),
+                            "rel_rmse": round(float(np.sqrt(((im - ref) ** 2).mean())) / norm, 5)})
+        ctx.program_destroy(p)
+        return out
+
+    cps = [1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024]
+    plain = run(0, 1024, cps)
+    cache = run(1, 1024, cps)
+    target = plain[-1]["rel_rmse"]
+    reach = next((c for c in cache if c["rel_rmse"] <= target), None)
+    result = {
+        "metric": "learned-cache time to the plain tracer's 1024-spp error, fraction of the plain time",
+        "value": round(reach["seconds"] / plain[-1]["seconds"], 4) if reach else None,
+        "unit": "fraction", "n_gpus": 1, "steps": 1024, "warmup": 0,
+        "ms_per_step": round(cache[-1]["seconds"] / 1024 * 1e3, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (network) / f32",
+        "data": "synthetic",
+        "config": {"workload": "10000 instances of 5 BLAS, %dx%d, 1 spp per frame, depth 8, RR from 3, NEE, "
+                               "gradient sky; cache: 64-64x4-3 ReLU MLP, BF16 MFMA, 4 Adam steps per frame on "
+                               "<= 65536 rows from 1-in-R training paths; reference 4096 spp plain" % (W, H),
+                   "parallelism": "single GPU"},
+        "extra": {"plain": plain, "cache": cache, "plain_1024spp_rel_rmse": target},
     }
     ctx.close()
     print(json.dumps(result))

@@ -42,6 +42,7 @@ typedef uint64_t solhip_blas;
 typedef uint64_t solhip_tlas;
 typedef uint64_t solhip_program;
 typedef uint64_t solhip_server;
+typedef uint64_t solhip_mlp;
 
 enum {
     SOLHIP_OK = 0,
@@ -169,7 +170,14 @@ typedef struct {
     uint32_t wf_block;        /* threads per block of the wavefront kernels (64..256); 0 = default */
     uint32_t wf_graph;        /* 1: replay the frame's passes as a captured hipGraph (recaptured
                                  when buffers, scene or launch shape change) */
-    uint32_t reserved[4];
+    /* learned radiance cache (SOLHIP_PROG_PATHTRACE): paths end at their first
+     * bounce vertex, where a network (solhip_mlp_*) trained online on the
+     * tracer's own training paths predicts the radiance leaving it.  Biased;
+     * the plain tracer (nrc = 0) is its reference.  Not captured in a graph
+     * (training reads its batch size back). */
+    uint32_t nrc;             /* 1: use the cache */
+    uint32_t nrc_train_steps; /* Adam steps per frame (0 = 4) */
+    uint32_t reserved[2];
 } solhip_spec;
 
 /* per-dispatch images: W*H pixels, row-major, no padding */
@@ -336,6 +344,28 @@ int32_t solhip_server_submit(solhip_ctx ctx, solhip_server srv, const solhip_job
  * queued or running afterwards */
 int32_t solhip_server_run(solhip_ctx ctx, solhip_server srv, uint32_t max_steps, uint32_t *live);
 int32_t solhip_server_job(solhip_ctx ctx, solhip_server srv, uint64_t job_id, solhip_job_status *out);
+
+/* ---- radiance-cache network (no counterpart in the reference) ----------
+ * A 64-input MLP, four hidden ReLU layers of 64, 3 linear outputs, no
+ * biases; BF16 MFMA with FP32 accumulation; Adam (beta1 0.9, beta2 0.99,
+ * eps 1e-8) on FP32 master weights; loss = relative L2,
+ * mean over rows of sum_c (p_c - t_c)^2 / (p_c^2 + 0.01) with the
+ * denominator held constant.  Parameters: 17408 floats, W_l[k][n] at
+ * l*4096 + k*64 + n (l = 0..3), W_4[k][n] at 16384 + k*16 + n (n < 3; the
+ * other columns are kept 0).  Inputs: n rows of 64 BF16; outputs and targets:
+ * n float4 (x, y, z, unused). */
+int32_t solhip_mlp_create(solhip_ctx ctx, uint32_t seed, float learning_rate, solhip_mlp *out);
+int32_t solhip_mlp_destroy(solhip_ctx ctx, solhip_mlp m);
+/* copies the FP32 parameters out (get, may be NULL) and/or in (set, may be NULL) */
+int32_t solhip_mlp_params(solhip_ctx ctx, solhip_mlp m, float *get, const float *set);
+int32_t solhip_mlp_forward(solhip_ctx ctx, solhip_stream s, solhip_mlp m, solhip_buffer in_bf16, uint32_t n,
+                           solhip_buffer out_f32x4);
+/* steps Adam steps on the full batch; *loss (may be NULL) = loss before the last step */
+int32_t solhip_mlp_train(solhip_ctx ctx, solhip_stream s, solhip_mlp m, solhip_buffer in_bf16,
+                         solhip_buffer target_f32x4, uint32_t n, uint32_t steps, float *loss);
+/* the loss gradient (17408 floats) of one batch without updating */
+int32_t solhip_mlp_gradient(solhip_ctx ctx, solhip_stream s, solhip_mlp m, solhip_buffer in_bf16,
+                            solhip_buffer target_f32x4, uint32_t n, float *grad, float *loss);
 
 #ifdef __cplusplus
 }

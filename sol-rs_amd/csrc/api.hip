@@ -167,6 +167,77 @@ int build_lights(hipStream_t s, const DevTables &tb, const Tlas *t, DevBuf &offs
     return SOLHIP_OK;
 }
 
+// the radiance-cache frame: buffers, scene box, the passes with cache
+// queries, then training on the frame's rows
+struct NrcHookCtx {
+    Program *p;
+    const WfArgs *w;
+};
+
+static int nrc_query_hook(void *user, hipStream_t s)
+{
+    NrcHookCtx *h = (NrcHookCtx *)user;
+    return mlp_query(s, h->p->nrc_mlp, h->w->nrc.qfeat, h->w->nrc.counts, h->w->nslots, h->w->nrc.qrec,
+                     h->w->st1);
+}
+
+int trace_nrc(Context *c, hipStream_t s, Program *p, Tlas *t, WfArgs &w, Timer *timer)
+{
+    if (!p->nrc_mlp) {
+        p->nrc_mlp = mlp_new(c, 1u, 1e-3f);
+        if (!p->nrc_mlp) return SOLHIP_ERROR_DEVICE;
+    }
+    NrcArgs &N = w.nrc;
+    N.enabled = 1;
+    N.max_rec = 4;
+    const uint64_t npix = (uint64_t)w.W * w.H;
+    N.batch_cap = 65536;
+    N.ratio = (uint32_t)std::max<uint64_t>(1, npix * w.spp * N.max_rec / N.batch_cap);
+    N.train_cap = (uint32_t)std::min<uint64_t>(npix, 2 * (npix / N.ratio) + 1024);
+    const uint64_t ns = w.nslots, tc = N.train_cap, nr = tc * N.max_rec;
+    if (p->nq_feat.reserve(ns * 128) || p->nq_rec.reserve(ns * sizeof(CacheQuery)) || p->n_counts.reserve(256) ||
+        p->n_tidx.reserve(npix * 4) || p->n_tpath.reserve(tc * 4) || p->n_trec_n.reserve(tc * 4) ||
+        p->n_tfeat.reserve(nr * 128) || p->n_trec.reserve(nr * 32) || p->n_bfeat.reserve((uint64_t)N.batch_cap * 128) ||
+        p->n_btarget.reserve((uint64_t)N.batch_cap * 16))
+        return SOLHIP_ERROR_OUT_OF_MEMORY;
+    N.qfeat = (uint16_t *)p->nq_feat.ptr;
+    N.qrec = (CacheQuery *)p->nq_rec.ptr;
+    N.counts = (uint32_t *)p->n_counts.ptr;
+    N.tidx = (uint32_t *)p->n_tidx.ptr;
+    N.tpath = (uint32_t *)p->n_tpath.ptr;
+    N.trec_n = (uint32_t *)p->n_trec_n.ptr;
+    N.tfeat = (uint16_t *)p->n_tfeat.ptr;
+    N.trec = (float4 *)p->n_trec.ptr;
+    N.bfeat = (uint16_t *)p->n_bfeat.ptr;
+    N.btarget = (float4 *)p->n_btarget.ptr;
+    // scene box from the TLAS root (world space)
+    float box[6] = {0, 0, 0, 1, 1, 1};
+    if (t->bvh.prim_count) {
+        SOLHIP_HIP_CHECK(hipMemcpyAsync(box, t->bvh.root_box.ptr, 24, hipMemcpyDeviceToHost, s));
+        SOLHIP_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    for (int a = 0; a < 3; ++a) {
+        const float ext = box[3 + a] - box[a];
+        N.lo[a] = box[a];
+        N.inv_ext[a] = ext > 0.0f ? 1.0f / ext : 1.0f;
+    }
+    int rc = launch_fill(s, N.counts, 4, 0u);
+    if (rc) return rc;
+    NrcHookCtx hc{p, &w};
+    PassHook hook{nrc_query_hook, &hc};
+    rc = launch_pathtrace(s, w, timer, &hook);
+    if (rc) return rc;
+    uint32_t rows = 0;
+    SOLHIP_HIP_CHECK(hipMemcpyAsync(&rows, N.counts + 2, 4, hipMemcpyDeviceToHost, s));
+    SOLHIP_HIP_CHECK(hipStreamSynchronize(s));
+    rows = std::min(rows, N.batch_cap);
+    const uint32_t steps = p->spec.nrc_train_steps ? p->spec.nrc_train_steps : 4u;
+    rc = mlp_train_dev(s, p->nrc_mlp, N.bfeat, N.btarget, rows, steps);
+    p->nrc_frames++;
+    p->nrc_rows += rows;
+    return rc;
+}
+
 // SOLHIP_PROG_PATHTRACE: buffers, emitter table, the wavefront passes
 int trace_wavefront(Context *c, hipStream_t s, Program *p, Tlas *t, const ProgArgs &a)
 {
@@ -230,6 +301,7 @@ int trace_wavefront(Context *c, hipStream_t s, Program *p, Tlas *t, const ProgAr
         w.lights = (const LightTri *)p->lights.ptr;
         w.nlights = nl;
     }
+    if (p->spec.nrc) return trace_nrc(c, s, p, t, w, timer);
     if (!p->spec.wf_graph || timer) return launch_pathtrace(s, w, timer);
     // replay the captured passes; recapture when anything the kernels were
     // launched with differs (buffers, scene pointers, sizes, spec)

@@ -88,6 +88,32 @@ struct FrameConsts {
     uint32_t pad[2];
 };
 
+// a radiance-cache query: path and its throughput at the query vertex
+struct CacheQuery {
+    uint32_t path;
+    float T[3];
+};
+
+// radiance cache state of a wavefront frame (null pointers when off)
+struct NrcArgs {
+    uint32_t enabled;
+    uint32_t ratio;          // one path in `ratio` is a training path
+    uint32_t max_rec;        // training records per path (vertices 1 .. max_rec)
+    uint32_t train_cap;      // training paths per pass
+    uint32_t batch_cap;      // training rows per frame
+    float lo[3], inv_ext[3]; // scene box -> [0, 1]^3
+    uint16_t *qfeat;         // query features [nslots][64] BF16
+    CacheQuery *qrec;
+    uint32_t *counts;        // [0] queries, [1] training paths, [2] batch rows
+    uint32_t *tidx;          // per pixel: training index or ~0
+    uint32_t *tpath;         // per training index: pixel
+    uint32_t *trec_n;        // per training index: records written
+    uint16_t *tfeat;         // [train_cap * max_rec][64] BF16
+    float4 *trec;            // [train_cap * max_rec][2]: (T, -), (L, -) on arrival
+    uint16_t *bfeat;         // training batch features [batch_cap][64]
+    float4 *btarget;         // training batch targets
+};
+
 struct WfArgs {
     SceneView sv;
     DevTables tb;
@@ -108,6 +134,7 @@ struct WfArgs {
     uint32_t *heads;                  // dynamic-fetch heads, 2 per bounce
     const LightTri *lights;
     uint32_t nlights;
+    NrcArgs nrc;
     uint32_t block;                   // threads per block
     uint32_t persist_blocks;          // grid of the extend / shadow kernels
     uint32_t shade_blocks;            // grid of the shade kernel
@@ -131,8 +158,15 @@ int launch_lights_count(hipStream_t s, const DevTables &tb, const uint32_t *inst
                         uint32_t *offsets, Timer *timer);
 int launch_lights_fill(hipStream_t s, const DevTables &tb, const uint32_t *offsets, uint32_t n_inst,
                        LightTri *lights, uint32_t nlights, Timer *timer);
-// one frame of the wavefront path tracer (kernels only: graph-capturable)
-int launch_pathtrace(hipStream_t s, const WfArgs &a, Timer *timer);
+// work the caller runs between the kernels of a pass (the cache queries
+// after the shade that ends query paths)
+struct PassHook {
+    int (*after_query_shade)(void *user, hipStream_t s);
+    void *user;
+};
+// one frame of the wavefront path tracer (kernels only: graph-capturable
+// when the radiance cache is off)
+int launch_pathtrace(hipStream_t s, const WfArgs &a, Timer *timer, const PassHook *hook = nullptr);
 // writes the frame constants through a kernel argument (no host buffer to keep alive)
 int launch_set_frame(hipStream_t s, FrameConsts *dst, const FrameConsts &value);
 

@@ -16,7 +16,16 @@
 
 namespace solhip {
 
-enum class Kind { Context, Stream, Buffer, Blas, Tlas, Program, Server };
+// radiance-cache network (mlp.hip)
+struct Mlp;
+struct Context;
+Mlp *mlp_new(Context *c, uint32_t seed, float lr);
+void mlp_delete(Mlp *m);
+int mlp_query(hipStream_t s, Mlp *m, const uint16_t *feat, const uint32_t *n_dev, uint32_t n_max,
+              const CacheQuery *qrec, float4 *st1);
+int mlp_train_dev(hipStream_t s, Mlp *m, const uint16_t *feat, const float4 *target, uint32_t n, uint32_t steps);
+
+enum class Kind { Context, Stream, Buffer, Blas, Tlas, Program, Server, Mlp };
 
 struct Obj {
     Kind kind;
@@ -106,6 +115,10 @@ struct Program : Obj {
     DevBuf queue, qcount;
     // SOLHIP_PROG_PATHTRACE: path state, queues, emitter table, frame constants
     DevBuf st0, st1, psum, q0, q1, hits, sq, scontrib, counts, heads, lights, light_offsets, fc;
+    // radiance cache (spec.nrc)
+    Mlp *nrc_mlp = nullptr;
+    DevBuf nq_feat, nq_rec, n_counts, n_tidx, n_tpath, n_trec_n, n_tfeat, n_trec, n_bfeat, n_btarget;
+    uint64_t nrc_frames = 0, nrc_rows = 0;
     // captured frame (wf_graph): valid for the WfArgs it was captured with
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -127,8 +140,11 @@ struct Program : Obj {
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
         cap_stream = nullptr;
         for (DevBuf *b : {&queue, &qcount, &st0, &st1, &psum, &q0, &q1, &hits, &sq, &scontrib, &counts, &heads,
-                          &lights, &light_offsets, &fc})
+                          &lights, &light_offsets, &fc, &nq_feat, &nq_rec, &n_counts, &n_tidx, &n_tpath, &n_trec_n,
+                          &n_tfeat, &n_trec, &n_bfeat, &n_btarget})
             b->release();
+        mlp_delete(nrc_mlp);
+        nrc_mlp = nullptr;
     }
 };
 

@@ -218,6 +218,195 @@ __global__ void k_pt_shade(WfArgs A, uint32_t b)
     }
 }
 
+// ---- radiance cache (NrcArgs) ---------------------------------------------
+//
+// Query paths end at their first bounce vertex (depth 1), where the network
+// predicts the radiance leaving the vertex towards the previous one; their
+// throughput times the prediction joins the path radiance.  One path in
+// `ratio` (per pass, by a hash of pixel and sample) is a training path: it
+// runs unbiased to the end and records, at each non-emitter vertex 1 ..
+// max_rec, the features with the throughput and radiance on arrival; after
+// the pass the radiance gathered from a vertex on, divided by the
+// throughput into it, is that vertex's training target.
+
+__device__ __forceinline__ uint16_t bf16_bits(float f)
+{
+    const __bf16 h = (__bf16)f;
+    return __builtin_bit_cast(uint16_t, h);
+}
+
+// 64 features: frequency encoding of the normalised position (6 octaves x
+// sin/cos x 3 = 36), shading normal, direction to the previous vertex,
+// albedo, roughness, metallic, a constant 1, zeros
+__device__ __forceinline__ void nrc_features(const NrcArgs &N, V3 pos, V3 nO, V3 d, V3 albedo, float roughness,
+                                             float metallic, uint16_t *row)
+{
+    float f[64];
+    const float pn[3] = {(pos.x - N.lo[0]) * N.inv_ext[0], (pos.y - N.lo[1]) * N.inv_ext[1],
+                         (pos.z - N.lo[2]) * N.inv_ext[2]};
+    int k = 0;
+    for (int a = 0; a < 3; ++a) {
+        float scale = kPi;
+        for (int o = 0; o < 6; ++o) {
+            float sn, cs;
+            own_sincos(pn[a] * scale, sn, cs);
+            f[k++] = sn;
+            f[k++] = cs;
+            scale = scale * 2.0f;
+        }
+    }
+    const V3 wo = neg(normalize(d));
+    f[k++] = nO.x;
+    f[k++] = nO.y;
+    f[k++] = nO.z;
+    f[k++] = wo.x;
+    f[k++] = wo.y;
+    f[k++] = wo.z;
+    f[k++] = albedo.x;
+    f[k++] = albedo.y;
+    f[k++] = albedo.z;
+    f[k++] = roughness;
+    f[k++] = metallic;
+    f[k++] = 1.0f;
+    while (k < 64) f[k++] = 0.0f;
+    for (int c = 0; c < 8; ++c) {
+        uint4 v;
+        v.x = bf16_bits(f[8 * c + 0]) | ((uint32_t)bf16_bits(f[8 * c + 1]) << 16);
+        v.y = bf16_bits(f[8 * c + 2]) | ((uint32_t)bf16_bits(f[8 * c + 3]) << 16);
+        v.z = bf16_bits(f[8 * c + 4]) | ((uint32_t)bf16_bits(f[8 * c + 5]) << 16);
+        v.w = bf16_bits(f[8 * c + 6]) | ((uint32_t)bf16_bits(f[8 * c + 7]) << 16);
+        reinterpret_cast<uint4 *>(row)[c] = v;
+    }
+}
+
+// training paths of this pass
+__global__ void k_nrc_select(WfArgs A, uint32_t pass)
+{
+    const NrcArgs &N = A.nrc;
+    const uint32_t npix = A.W * A.H, lane = threadIdx.x & 63u, nthreads = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < npix; base += nthreads) {
+        const uint32_t p = base + lane;
+        const bool sel = p < npix && tea(p, A.fc->frame * A.spp + pass) % N.ratio == 0u;
+        const uint32_t idx = wave_append(N.counts + 1, sel);
+        if (p < npix) {
+            const bool ok = sel && idx < N.train_cap;
+            N.tidx[p] = ok ? idx : 0xFFFFFFFFu;
+            if (ok) {
+                N.tpath[idx] = p;
+                N.trec_n[idx] = 0;
+            }
+        }
+    }
+}
+
+__global__ void k_pt_shade_nrc(WfArgs A, uint32_t b)
+{
+    const uint32_t cnt = A.counts[b & 1];
+    const QRay *qin = A.q[b & 1];
+    QRay *qout = A.q[(b + 1) & 1];
+    uint32_t *cout = A.counts + ((b + 1) & 1);
+    const NrcArgs &N = A.nrc;
+    const uint32_t lane = threadIdx.x & 63u, nthreads = gridDim.x * blockDim.x;
+    ShadeParams P;
+    P.tb = A.tb;
+    P.enable_sky = A.enable_sky;
+    P.max_depth = A.max_depth;
+    P.rr_start = A.rr_start;
+    P.lights = A.lights;
+    P.nlights = A.nlights;
+    P.total = A.nlights ? A.lights[A.nlights - 1].cdf : 0.0f;
+    P.nee = A.nee && A.nlights > 0 && P.total > 0.0f;
+    P.rmax = A.rmax;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < cnt; base += nthreads) {
+        const uint32_t i = base + lane;
+        ShadeOut o;
+        bool query = false;
+        CacheQuery qr;
+        uint16_t qf[64];
+        if (i < cnt) {
+            const QRay r = qin[i];
+            if (r.path != kNoPath) {
+                const uint32_t p = r.path;
+                const solhip_hit h = A.hits[i];
+                float4 s0 = A.st0[p], s1 = A.st1[p];
+                const uint32_t depth = as_u(s1.w) & 0xFFFFu;
+                bool shade = true;
+                if (h.inst != SOLHIP_MISS && depth >= 1) {
+                    const float *mat = reinterpret_cast<const float *>(A.tb.material_table[h.inst]);
+                    const uint32_t ti = N.tidx[p];
+                    const bool record = ti != 0xFFFFFFFFu && depth <= N.max_rec;
+                    const bool ask = ti == 0xFFFFFFFFu && depth == 1;
+                    if (!is_emitter(mat) && (record || ask)) {
+                        const Surf sf = fetch_surface(A.tb, h.inst, h.prim, h.u, h.v);
+                        const V3 d = v3(r.dx, r.dy, r.dz);
+                        const V3 wI = normalize(d);
+                        const V3 nO = scale(sf.nrm, sgn(dot(sf.nrm, neg(wI))));
+                        const V3 alb = v3(mat[0] * sf.col.x, mat[1] * sf.col.y, mat[2] * sf.col.z);
+                        if (ask) {
+                            nrc_features(N, sf.pos, nO, d, alb, mat[9], mat[8], qf);
+                            qr.path = p;
+                            qr.T[0] = s0.x;
+                            qr.T[1] = s0.y;
+                            qr.T[2] = s0.z;
+                            query = true;
+                            shade = false;   // the cache answers for the rest of the path
+                        } else {
+                            const uint32_t rec = ti * N.max_rec + (depth - 1);
+                            nrc_features(N, sf.pos, nO, d, alb, mat[9], mat[8], N.tfeat + (uint64_t)rec * 64);
+                            N.trec[2ull * rec] = make_float4(s0.x, s0.y, s0.z, 0.0f);
+                            N.trec[2ull * rec + 1] = make_float4(s1.x, s1.y, s1.z, 0.0f);
+                            N.trec_n[ti] = depth;
+                        }
+                    }
+                }
+                if (shade) {
+                    shade_vertex(P, p, r, h, s0, s1, o);
+                    A.st0[p] = s0;
+                    A.st1[p] = s1;
+                }
+            }
+        }
+        const uint32_t s_next = wave_append(cout, o.cont);
+        if (o.cont) qout[s_next] = o.nr;
+        const uint32_t s_sh = wave_append(A.counts + 2, o.shadow);
+        if (o.shadow) {
+            A.sq[s_sh] = o.sr;
+            A.scontrib[s_sh] = o.sc;
+        }
+        const uint32_t s_q = wave_append(N.counts + 0, query);
+        if (query) {
+            N.qrec[s_q] = qr;
+            uint4 *dst = reinterpret_cast<uint4 *>(N.qfeat + (uint64_t)s_q * 64);
+            for (int c = 0; c < 8; ++c) dst[c] = reinterpret_cast<const uint4 *>(qf)[c];
+        }
+    }
+}
+
+// training rows of this pass: (features, radiance from the vertex on / throughput into it)
+__global__ void k_nrc_targets(WfArgs A)
+{
+    const NrcArgs &N = A.nrc;
+    const uint32_t ntrain = min(N.counts[1], N.train_cap);
+    const uint32_t nthreads = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ntrain; i += nthreads) {
+        const float4 Lf = A.st1[N.tpath[i]];
+        const uint32_t nrec = N.trec_n[i];
+        for (uint32_t k = 0; k < nrec; ++k) {
+            const uint64_t rec = (uint64_t)i * N.max_rec + k;
+            const float4 T = N.trec[2 * rec], L = N.trec[2 * rec + 1];
+            if (!(T.x > 0.0f && T.y > 0.0f && T.z > 0.0f)) continue;
+            const float4 tg = make_float4((Lf.x - L.x) / T.x, (Lf.y - L.y) / T.y, (Lf.z - L.z) / T.z, 0.0f);
+            if (!(isfinite(tg.x) && isfinite(tg.y) && isfinite(tg.z))) continue;
+            const uint32_t slot = atomicAdd(N.counts + 2, 1u);
+            if (slot >= N.batch_cap) continue;
+            const uint4 *src = reinterpret_cast<const uint4 *>(N.tfeat + rec * 64);
+            uint4 *dst = reinterpret_cast<uint4 *>(N.bfeat + (uint64_t)slot * 64);
+            for (int c = 0; c < 8; ++c) dst[c] = src[c];
+            N.btarget[slot] = tg;
+        }
+    }
+}
+
 __global__ void k_pt_finalize(WfArgs A)
 {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
@@ -386,7 +575,7 @@ int launch_set_frame(hipStream_t s, FrameConsts *dst, const FrameConsts &value)
     return launch_status("frame constants");
 }
 
-int launch_pathtrace(hipStream_t s, const WfArgs &a, Timer *timer)
+int launch_pathtrace(hipStream_t s, const WfArgs &a, Timer *timer, const PassHook *hook)
 {
     const uint32_t blk = a.block;
     const size_t lds = (size_t)a.sv.stack_depth * sizeof(uint2) * blk;
@@ -399,16 +588,28 @@ int launch_pathtrace(hipStream_t s, const WfArgs &a, Timer *timer)
         launch();
         if (timer) timer->end(s, id);
     };
+    const bool nrc = a.nrc.enabled != 0;
     for (uint32_t pass = 0; pass < a.spp; ++pass) {
         timed(SOLHIP_KERNEL_PT_GENERATE,
               [&] { hipLaunchKernelGGL(k_pt_generate, dim3(gen_blocks), dim3(blk), 0, s, a, pass); });
+        if (nrc) {
+            int rc = launch_fill(s, a.nrc.counts, 2, 0u);   // queries, training paths
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_nrc_select, dim3(a.shade_blocks), dim3(blk), 0, s, a, pass);
+        }
         for (uint32_t b = 0; b <= a.max_depth; ++b) {
             timed(SOLHIP_KERNEL_PT_EXTEND, [&] {
                 if (a.count) hipLaunchKernelGGL(k_pt_extend<true>, dim3(a.persist_blocks), dim3(blk), lds, s, a, b);
                 else hipLaunchKernelGGL(k_pt_extend<false>, dim3(a.persist_blocks), dim3(blk), lds, s, a, b);
             });
-            timed(SOLHIP_KERNEL_PT_SHADE,
-                  [&] { hipLaunchKernelGGL(k_pt_shade, dim3(a.shade_blocks), dim3(blk), 0, s, a, b); });
+            timed(SOLHIP_KERNEL_PT_SHADE, [&] {
+                if (nrc) hipLaunchKernelGGL(k_pt_shade_nrc, dim3(a.shade_blocks), dim3(blk), 0, s, a, b);
+                else hipLaunchKernelGGL(k_pt_shade, dim3(a.shade_blocks), dim3(blk), 0, s, a, b);
+            });
+            if (nrc && b == 1 && hook) {
+                int rc = hook->after_query_shade(hook->user, s);
+                if (rc) return rc;
+            }
             if (shadows && b < a.max_depth) {
                 timed(SOLHIP_KERNEL_PT_SHADOW, [&] {
                     if (a.count) hipLaunchKernelGGL(k_pt_shadow<true>, dim3(a.persist_blocks), dim3(blk), lds, s, a, b);
@@ -416,6 +617,7 @@ int launch_pathtrace(hipStream_t s, const WfArgs &a, Timer *timer)
                 });
             }
         }
+        if (nrc) hipLaunchKernelGGL(k_nrc_targets, dim3(a.shade_blocks), dim3(blk), 0, s, a);
         int rc = launch_status("pathtrace pass");
         if (rc) return rc;
     }

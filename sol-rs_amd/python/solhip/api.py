@@ -44,7 +44,8 @@ class Spec(C.Structure):
     _fields_ = [("enable_sky", C.c_uint32), ("spp_per_frame", C.c_uint32), ("max_depth", C.c_uint32),
                 ("n_ao", C.c_uint32), ("ao_tmin", C.c_float), ("ao_tmax", C.c_float), ("ao_any_hit", C.c_uint32),
                 ("rr_start", C.c_uint32), ("nee", C.c_uint32), ("wf_blocks_per_cu", C.c_uint32),
-                ("wf_block", C.c_uint32), ("wf_graph", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+                ("wf_block", C.c_uint32), ("wf_graph", C.c_uint32), ("nrc", C.c_uint32),
+                ("nrc_train_steps", C.c_uint32), ("reserved", C.c_uint32 * 2)]
 
 
 class Images(C.Structure):
@@ -126,6 +127,12 @@ SIGNATURES = {
     "solhip_server_submit": (_i32, [_u64, _u64, C.POINTER(JobDesc), C.POINTER(_u64)]),
     "solhip_server_run": (_i32, [_u64, _u64, _u32, C.POINTER(_u32)]),
     "solhip_server_job": (_i32, [_u64, _u64, _u64, C.POINTER(JobStatus)]),
+    "solhip_mlp_create": (_i32, [_u64, _u32, C.c_float, C.POINTER(_u64)]),
+    "solhip_mlp_destroy": (_i32, [_u64, _u64]),
+    "solhip_mlp_params": (_i32, [_u64, _u64, _vp, _vp]),
+    "solhip_mlp_forward": (_i32, [_u64, _u64, _u64, _u64, _u32, _u64]),
+    "solhip_mlp_train": (_i32, [_u64, _u64, _u64, _u64, _u64, _u32, _u32, C.POINTER(C.c_float)]),
+    "solhip_mlp_gradient": (_i32, [_u64, _u64, _u64, _u64, _u64, _u32, _vp, C.POINTER(C.c_float)]),
 }
 
 _lib = None
@@ -404,5 +411,47 @@ class Server:
     def close(self):
         if self.h:
             check(lib().solhip_server_destroy(self.ctx.h, self.h))
+            self.h = 0
+
+
+MLP_PARAMS = 17408
+
+
+class Mlp:
+    """the radiance-cache network (solhip_mlp_*): 64 -> 4 x 64 ReLU -> 3"""
+
+    def __init__(self, ctx, seed=1, lr=1e-2):
+        self.ctx = ctx
+        h = C.c_uint64()
+        check(lib().solhip_mlp_create(ctx.h, seed, lr, C.byref(h)))
+        self.h = h.value
+
+    def params(self, set_to=None):
+        out = np.zeros(MLP_PARAMS, np.float32)
+        if set_to is not None:
+            w = np.ascontiguousarray(set_to, np.float32)
+            assert w.size == MLP_PARAMS
+            check(lib().solhip_mlp_params(self.ctx.h, self.h, _ptr(out), _ptr(w)))
+        else:
+            check(lib().solhip_mlp_params(self.ctx.h, self.h, _ptr(out), None))
+        return out
+
+    def forward(self, x_buf, n, y_buf, stream=0):
+        check(lib().solhip_mlp_forward(self.ctx.h, stream, self.h, x_buf.h, n, y_buf.h))
+
+    def train(self, x_buf, t_buf, n, steps=1, stream=0):
+        loss = C.c_float(0.0)
+        check(lib().solhip_mlp_train(self.ctx.h, stream, self.h, x_buf.h, t_buf.h, n, steps, C.byref(loss)))
+        return float(loss.value)
+
+    def gradient(self, x_buf, t_buf, n, stream=0):
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_float(0.0)
+        check(lib().solhip_mlp_gradient(self.ctx.h, stream, self.h, x_buf.h, t_buf.h, n, _ptr(g), C.byref(loss)))
+        return g, float(loss.value)
+
+    def close(self):
+        if self.h:
+            check(lib().solhip_mlp_destroy(self.ctx.h, self.h))
             self.h = 0
 
