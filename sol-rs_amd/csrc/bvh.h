@@ -30,7 +30,7 @@ static_assert(sizeof(BlasRef) == 32, "BlasRef layout");
 // build scratch, kept by each BLAS / TLAS so that rebuilds allocate nothing
 struct DevScratch {
     DevBuf boxes, bounds, keys[2], vals[2], hist, left, right, parent, rfirst, flags, nbox, ncount, counters,
-        wsrc, leaf_ids;
+        wsrc, leaf_ids, frontier;
     std::vector<DevBuf> scan_bufs;
     uint32_t *pinned = nullptr;
     int ensure(uint32_t n);

@@ -249,25 +249,49 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const uint32_t *keys_in, con
 
 // ---- Karras 2012 radix tree ---------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_karras(const uint32_t *keys, int n, uint32_t *left, uint32_t *right,
-                                                   uint32_t *parent, uint32_t *rfirst)
+                                                   uint32_t *parent, uint32_t *rfirst, uint32_t *ncount)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n - 1) return;
-    karras_node(keys, n, i, left, right, parent, rfirst);
+    karras_node(keys, n, i, left, right, parent, rfirst, ncount);
     if (i == 0) parent[0] = 0xFFFFFFFFu;
 }
 
-// bottom-up boxes; nbox[2*node] = lo, [2*node+1] = hi; ncount = primitives
-__global__ void __launch_bounds__(kBlock) k_refit(const float4 *pbox, const uint32_t *vals, int n,
-                                                  const uint32_t *left, const uint32_t *right,
-                                                  const uint32_t *parent, uint32_t *flags, float4 *nbox,
-                                                  uint32_t *ncount, uint32_t *err)
+// bottom-up boxes; nbox[2*node] = lo, [2*node+1] = hi (ncount of internal
+// nodes comes from k_karras: the size of the node's leaf range).
+//
+// Two phases.  k_refit_local: block b owns sorted leaves [256b, 256b+256);
+// an internal node whose leaf range lies inside that span has its index in
+// it too (one end of a radix-tree node's range is the node itself), so the
+// block builds those nodes with workgroup-scope LDS flags and LDS boxes (the
+// second child to arrive merges).  A node whose parent's range leaves the
+// span is a frontier node: k_refit_cross walks up from the frontier nodes
+// with agent-scope flags, over the few nodes that span blocks.
+constexpr uint32_t kRefitSpan = kBlock;
+
+__device__ __forceinline__ void box_union(float4 &lo, float4 &hi, float4 blo, float4 bhi)
 {
-    const int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= n) return;
-    uint32_t node = (uint32_t)(n - 1 + k);
-    const uint32_t p = vals[k];
-    float4 lo = pbox[2ull * p], hi = pbox[2ull * p + 1];
+    lo = make_float4(fminf(lo.x, blo.x), fminf(lo.y, blo.y), fminf(lo.z, blo.z), 0.0f);
+    hi = make_float4(fmaxf(hi.x, bhi.x), fmaxf(hi.y, bhi.y), fmaxf(hi.z, bhi.z), 0.0f);
+}
+
+__global__ void __launch_bounds__(kBlock) k_refit_local(const float4 *pbox, const uint32_t *vals, int n,
+                                                        const uint32_t *left, const uint32_t *right,
+                                                        const uint32_t *parent, const uint32_t *rfirst,
+                                                        uint32_t *ncount, float4 *nbox, uint32_t *frontier,
+                                                        uint32_t *nfrontier, uint32_t *err)
+{
+    __shared__ uint32_t lflag[kRefitSpan];
+    __shared__ float4 lbox[kRefitSpan][2];
+    const uint32_t L0 = blockIdx.x * kRefitSpan, L1 = min((uint32_t)n, L0 + kRefitSpan);
+    lflag[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t k = L0 + threadIdx.x;
+    if (k >= L1) return;
+    const uint32_t leaf_base = (uint32_t)(n - 1);
+    uint32_t node = leaf_base + k;
+    const uint32_t pk = vals[k];
+    float4 lo = pbox[2ull * pk], hi = pbox[2ull * pk + 1];
     nbox[2ull * node] = lo;
     nbox[2ull * node + 1] = hi;
     ncount[node] = 1;
@@ -275,25 +299,75 @@ __global__ void __launch_bounds__(kBlock) k_refit(const float4 *pbox, const uint
     while (true) {
         const uint32_t par = parent[node];
         if (par == 0xFFFFFFFFu) break;
-        if (par >= (uint32_t)(n - 1)) {
+        if (par >= leaf_base) {
             atomicAdd(err, 1u);
             break;
         }
-        const uint32_t old = __hip_atomic_fetch_add(&flags[par], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t plo = rfirst[par], phi = plo + ncount[par] - 1u;
+        if (plo < L0 || phi >= L1) {
+            frontier[atomicAdd(nfrontier, 1u)] = node;
+            break;
+        }
+        const uint32_t old = __hip_atomic_fetch_add(&lflag[par - L0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
         if (old == 0) break;
         const uint32_t l = left[par], r = right[par];
-        if (l >= 2u * (uint32_t)n - 1u || r >= 2u * (uint32_t)n - 1u) {
-            atomicAdd(err, 1u);
-            break;
+        const uint32_t sib = l == node ? r : l;
+        float4 slo, shi;
+        if (sib >= leaf_base) {
+            if (sib - leaf_base >= (uint32_t)n) {
+                atomicAdd(err, 1u);
+                break;
+            }
+            const uint32_t ps = vals[sib - leaf_base];
+            slo = pbox[2ull * ps];
+            shi = pbox[2ull * ps + 1];
+        } else {
+            if (sib < L0 || sib >= L1) {
+                atomicAdd(err, 1u);
+                break;
+            }
+            slo = lbox[sib - L0][0];
+            shi = lbox[sib - L0][1];
         }
-        const float4 al = nbox[2ull * l], ah = nbox[2ull * l + 1];
-        const float4 bl = nbox[2ull * r], bh = nbox[2ull * r + 1];
-        lo = make_float4(fminf(al.x, bl.x), fminf(al.y, bl.y), fminf(al.z, bl.z), 0.0f);
-        hi = make_float4(fmaxf(ah.x, bh.x), fmaxf(ah.y, bh.y), fmaxf(ah.z, bh.z), 0.0f);
+        box_union(lo, hi, slo, shi);
+        lbox[par - L0][0] = lo;
+        lbox[par - L0][1] = hi;
         nbox[2ull * par] = lo;
         nbox[2ull * par + 1] = hi;
-        ncount[par] = ncount[l] + ncount[r];
         node = par;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_refit_cross(int n, const uint32_t *left, const uint32_t *right,
+                                                        const uint32_t *parent, uint32_t *flags, float4 *nbox,
+                                                        const uint32_t *frontier, const uint32_t *nfrontier,
+                                                        uint32_t *err)
+{
+    const uint32_t count = min(*nfrontier, (uint32_t)n);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < count; f += stride) {
+        uint32_t node = frontier[f];
+        float4 lo = nbox[2ull * node], hi = nbox[2ull * node + 1];
+        while (true) {
+            const uint32_t par = parent[node];
+            if (par == 0xFFFFFFFFu) break;
+            if (par >= (uint32_t)(n - 1)) {
+                atomicAdd(err, 1u);
+                break;
+            }
+            const uint32_t old = __hip_atomic_fetch_add(&flags[par], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (old == 0) break;
+            const uint32_t l = left[par], r = right[par];
+            if (l >= 2u * (uint32_t)n - 1u || r >= 2u * (uint32_t)n - 1u) {
+                atomicAdd(err, 1u);
+                break;
+            }
+            const uint32_t sib = l == node ? r : l;
+            box_union(lo, hi, nbox[2ull * sib], nbox[2ull * sib + 1]);
+            nbox[2ull * par] = lo;
+            nbox[2ull * par + 1] = hi;
+            node = par;
+        }
     }
 }
 
@@ -440,8 +514,8 @@ static int build_common(hipStream_t s, uint32_t n, uint32_t leaf_max, DevScratch
     uint32_t *counters = (uint32_t *)sc.counters.ptr;
     uint32_t *pinned = sc.pinned;
     pinned[8] = 1u;
-    pinned[9] = pinned[10] = pinned[11] = pinned[12] = 0u;
-    SOLHIP_HIP_CHECK(hipMemcpyAsync(counters, pinned + 8, 20, hipMemcpyHostToDevice, s));
+    pinned[9] = pinned[10] = pinned[11] = pinned[12] = pinned[13] = 0u;
+    SOLHIP_HIP_CHECK(hipMemcpyAsync(counters, pinned + 8, 24, hipMemcpyHostToDevice, s));
     // Morton codes
     hipLaunchKernelGGL(k_morton, dim3(g), dim3(kBlock), 0, s, (const float4 *)sc.boxes.ptr, n,
                        (const uint32_t *)sc.bounds.ptr, (uint32_t *)sc.keys[0].ptr, (uint32_t *)sc.vals[0].ptr);
@@ -468,11 +542,15 @@ static int build_common(hipStream_t s, uint32_t n, uint32_t leaf_max, DevScratch
     uint32_t *ncount = (uint32_t *)sc.ncount.ptr;
     if (n > 1) {
         hipLaunchKernelGGL(k_karras, dim3(cdiv(n - 1, kBlock)), dim3(kBlock), 0, s, keys, (int)n, left, right, parent,
-                           rfirst);
+                           rfirst, ncount);
         hipLaunchKernelGGL(k_fill_u32_n, dim3(cdiv(n - 1, kBlock)), dim3(kBlock), 0, s, flags, n - 1, 0u);
     }
-    hipLaunchKernelGGL(k_refit, dim3(g), dim3(kBlock), 0, s, (const float4 *)sc.boxes.ptr, vals, (int)n, left, right,
-                       parent, flags, nbox, ncount, counters + 3);
+    uint32_t *frontier = (uint32_t *)sc.frontier.ptr;
+    hipLaunchKernelGGL(k_refit_local, dim3(cdiv(n, kRefitSpan)), dim3(kBlock), 0, s, (const float4 *)sc.boxes.ptr,
+                       vals, (int)n, left, right, parent, rfirst, ncount, nbox, frontier, counters + 5, counters + 3);
+    if (n > 1)
+        hipLaunchKernelGGL(k_refit_cross, dim3(std::min<uint32_t>(cdiv(n, kBlock), 2048)), dim3(kBlock), 0, s, (int)n,
+                           left, right, parent, flags, nbox, frontier, counters + 5, counters + 3);
     // binary root: internal node 0, or leaf 0 (node index n-1 = 0) when n == 1
     const uint32_t root = 0u;
     hipLaunchKernelGGL(k_root_box, dim3(1), dim3(64), 0, s, (const float4 *)nbox, root, (float *)out.root_box.ptr);
@@ -554,6 +632,7 @@ int DevScratch::ensure(uint32_t n)
     rc |= counters.reserve(256);
     rc |= wsrc.reserve(N * 4);
     rc |= leaf_ids.reserve(N * 4);
+    rc |= frontier.reserve(N * 4);
     if (!pinned) {
         if (hipHostMalloc((void **)&pinned, 256, hipHostMallocDefault) != hipSuccess) {
             pinned = nullptr;
@@ -567,7 +646,7 @@ int DevScratch::ensure(uint32_t n)
 void DevScratch::release()
 {
     for (DevBuf *b : {&boxes, &bounds, &keys[0], &keys[1], &vals[0], &vals[1], &hist, &left, &right, &parent, &rfirst,
-                      &flags, &nbox, &ncount, &counters, &wsrc, &leaf_ids})
+                      &flags, &nbox, &ncount, &counters, &wsrc, &leaf_ids, &frontier})
         b->release();
     for (auto &b : scan_bufs) b.release();
     scan_bufs.clear();

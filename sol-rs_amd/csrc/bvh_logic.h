@@ -47,7 +47,7 @@ SOLHIP_HD int karras_delta(const uint32_t *k, int n, int i, int j)
 // internal node i in [0, n-2] of the radix tree over n sorted codes; leaf k
 // is node n-1+k.  Writes children, parents and the covered leaf range.
 SOLHIP_HD void karras_node(const uint32_t *keys, int n, int i, uint32_t *left, uint32_t *right, uint32_t *parent,
-                           uint32_t *rfirst)
+                           uint32_t *rfirst, uint32_t *rcount = nullptr)
 {
     const int d = (karras_delta(keys, n, i, i + 1) - karras_delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
     const int dmin = karras_delta(keys, n, i, i - d);
@@ -73,6 +73,7 @@ SOLHIP_HD void karras_node(const uint32_t *keys, int n, int i, uint32_t *left, u
     parent[lc] = (uint32_t)i;
     parent[rc] = (uint32_t)i;
     rfirst[i] = (uint32_t)lo;
+    if (rcount) rcount[i] = (uint32_t)(hi - lo + 1);
 }
 
 struct CollapseArgs {
