@@ -22,7 +22,7 @@ namespace solhip {
 // minimum waves per SIMD requested for the AO ray kernel: it is VALU-issue
 // bound, so the register budget is traded against latency hiding
 #ifndef SOLHIP_AO_WAVES
-#define SOLHIP_AO_WAVES 4
+#define SOLHIP_AO_WAVES 6
 #endif
 #define SOLHIP_AO_RAYS_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SOLHIP_AO_WAVES)))
 
