@@ -871,10 +871,13 @@ int32_t solhip_trace(solhip_ctx ctx, solhip_stream st, solhip_program ph, solhip
     }
     switch (p->kind_id) {
     case SOLHIP_PROG_AO: {
-        if (p->queue.reserve(npix * sizeof(AoEntry)) || p->qcount.reserve(256)) return SOLHIP_ERROR_OUT_OF_MEMORY;
+        if (p->queue.reserve(npix * sizeof(AoEntry)) || p->qcount.reserve(256) || p->occl.reserve(npix * a.n_ao))
+            return SOLHIP_ERROR_OUT_OF_MEMORY;
         a.queue = (AoEntry *)p->queue.ptr;
         a.qcount = (uint32_t *)p->qcount.ptr;
-        int rc = launch_fill(s, a.qcount, 1, 0u);
+        a.occl = (uint8_t *)p->occl.ptr;
+        a.ncu = c->ncu;
+        int rc = launch_fill(s, a.qcount, 2, 0u);
         if (rc) return rc;
         return launch_ao(s, a, c->timer_or_null());
     }

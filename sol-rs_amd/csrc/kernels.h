@@ -53,7 +53,9 @@ struct ProgArgs {
     float ao_tmin, ao_tmax;
     uint32_t ao_any;
     AoEntry *queue;
-    uint32_t *qcount;
+    uint32_t *qcount;       // [0] queue entries, [1] ray cursor of the persistent AO kernel
+    uint8_t *occl;          // per AO ray: occluded (persistent AO kernel)
+    uint32_t ncu;
     // pathtrace
     uint32_t enable_sky, spp, max_depth;
     Counters *counters;   // per-kernel counter slots [SOLHIP_KERNEL_COUNT]

@@ -112,7 +112,7 @@ struct Tlas : Obj {
 struct Program : Obj {
     uint32_t kind_id = 0;
     solhip_spec spec{};
-    DevBuf queue, qcount;
+    DevBuf queue, qcount, occl;
     // SOLHIP_PROG_PATHTRACE: path state, queues, emitter table, frame constants
     DevBuf st0, st1, psum, q0, q1, hits, sq, scontrib, counts, heads, lights, light_offsets, fc;
     // radiance cache (spec.nrc)
@@ -139,7 +139,7 @@ struct Program : Obj {
         drop_graph();
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
         cap_stream = nullptr;
-        for (DevBuf *b : {&queue, &qcount, &st0, &st1, &psum, &q0, &q1, &hits, &sq, &scontrib, &counts, &heads,
+        for (DevBuf *b : {&queue, &qcount, &occl, &st0, &st1, &psum, &q0, &q1, &hits, &sq, &scontrib, &counts, &heads,
                           &lights, &light_offsets, &fc, &nq_feat, &nq_rec, &n_counts, &n_tidx, &n_tpath, &n_trec_n,
                           &n_tfeat, &n_trec, &n_bfeat, &n_btarget})
             b->release();

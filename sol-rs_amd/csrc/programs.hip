@@ -16,6 +16,7 @@
 #include "kernels.h"
 #include "shading.h"
 #include "traverse.h"
+#include "traverse_step.h"
 
 namespace solhip {
 
@@ -25,6 +26,11 @@ namespace solhip {
 #define SOLHIP_AO_WAVES 6
 #endif
 #define SOLHIP_AO_RAYS_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SOLHIP_AO_WAVES)))
+// persistent AO rays: a wave takes new rays for its idle lanes once at least
+// this many are idle (0: one ray per lane, no refill)
+#ifndef SOLHIP_AO_REFILL
+#define SOLHIP_AO_REFILL 16
+#endif
 
 // ---------------------------------------------------------------------------
 // plain ray-traced AO (wavefront)
@@ -140,6 +146,73 @@ __global__ void __launch_bounds__(kProgBlock) SOLHIP_AO_RAYS_OCCUPANCY k_ao_rays
     count_rays<COUNT>(A.counters + SOLHIP_KERNEL_AO_RAYS, active ? 1u : 0u, tc);
 }
 
+// Persistent AO rays: each lane runs the resumable traversal and, when its
+// ray is done, records the occlusion and (once enough lanes of the wave are
+// idle) takes the next ray from a global cursor, so incoherent rays do not
+// leave most of a wave waiting for its slowest lane.  Ray r = (queue entry
+// r / n_ao, AO ray r % n_ao), with the same direction as k_ao_rays.
+template <bool ANY, bool COUNT>
+__global__ void __launch_bounds__(kProgBlock) SOLHIP_AO_RAYS_OCCUPANCY k_ao_rays_persist(ProgArgs A)
+{
+    extern __shared__ uint2 stack[];
+    const uint32_t count = *A.qcount;
+    const uint32_t total = count * A.n_ao;
+    const uint32_t shift = __builtin_ctz(A.n_ao);
+    const uint32_t lane = threadIdx.x & 63u;
+    TravCounts tc{0, 0, 0, false};
+    uint32_t rays = 0;
+    TravState st;
+    bool has = false, exhausted = false;
+    uint32_t rid = 0, pix = 0;
+    while (true) {
+        const uint64_t idle = __ballot(!has);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (!exhausted && nidle >= (uint32_t)SOLHIP_AO_REFILL) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(A.qcount + 1, nidle);
+            base = __shfl(base, 0);
+            exhausted = base + nidle >= total;
+            if (!has) {
+                const uint32_t my = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                if (my < total) {
+                    const AoEntry e = A.queue[my >> shift];
+                    uint32_t rng = pcg_advance(e.rng, 2u * (my & (A.n_ao - 1u)));
+                    const float xi0 = next_rand(rng);
+                    const float xi1 = next_rand(rng);
+                    const V3 dir = sample_cosine(v3(e.nx, e.ny, e.nz), xi0, xi1);
+                    trav_init(st, A.sv, v3(e.px, e.py, e.pz), dir, A.ao_tmin, A.ao_tmax);
+                    rid = my;
+                    pix = e.pixel;
+                    has = true;
+                }
+            }
+        }
+        if (__ballot(has) == 0) break;
+        if (has && trav_step<ANY, COUNT>(st, A.sv, 0xFFu, stack + threadIdx.x, kProgBlock, tc)) {
+            const Hit h = trav_result(st);
+            A.occl[rid] = h.inst != SOLHIP_MISS ? 1u : 0u;
+            if (A.hits_out) store_hit(A.hits_out + (uint64_t)A.W * A.H + (uint64_t)pix * A.n_ao + (rid & (A.n_ao - 1u)), h);
+            ++rays;
+            has = false;
+        }
+    }
+    count_rays<COUNT>(A.counters + SOLHIP_KERNEL_AO_RAYS, rays, tc);
+}
+
+// one thread per queue entry: the AO value from its rays' occlusion bytes
+__global__ void k_ao_resolve(ProgArgs A)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= *A.qcount) return;
+    const uint8_t *o = A.occl + (uint64_t)q * A.n_ao;
+    uint32_t occ = 0;
+    for (uint32_t k = 0; k < A.n_ao; ++k) occ += o[k];
+    const float value = 1.0f - (float)occ / (float)A.n_ao;
+    float4 *px = A.accum + A.queue[q].pixel;
+    const float4 old = *px;
+    *px = make_float4(mixf(old.x, value, A.alpha), mixf(old.y, value, A.alpha), mixf(old.z, value, A.alpha), 1.0f);
+}
+
 // dynamic LDS: the traversal stack, sv.stack_depth 8-byte entries per lane
 static size_t stack_bytes(const ProgArgs &a) { return (size_t)a.sv.stack_depth * sizeof(uint2) * kProgBlock; }
 
@@ -167,9 +240,29 @@ int launch_ao(hipStream_t s, const ProgArgs &a, Timer *timer)
 {
     const uint32_t tiles = tile_count(a.W, a.H);
     SOLHIP_LAUNCH(k_ao_primary, SOLHIP_KERNEL_AO_PRIMARY, tiles, a, s, timer);
-    const uint64_t lanes = (uint64_t)a.W * a.H * a.n_ao;
-    const uint32_t blocks = (uint32_t)((lanes + kProgBlock - 1) / kProgBlock);
-    SOLHIP_LAUNCH(k_ao_rays, SOLHIP_KERNEL_AO_RAYS, blocks, a, s, timer);
+    if (SOLHIP_AO_REFILL == 0 || !a.occl) {
+        const uint64_t lanes = (uint64_t)a.W * a.H * a.n_ao;
+        const uint32_t blocks = (uint32_t)((lanes + kProgBlock - 1) / kProgBlock);
+        SOLHIP_LAUNCH(k_ao_rays, SOLHIP_KERNEL_AO_RAYS, blocks, a, s, timer);
+        return launch_status("AO");
+    }
+    // persistent grid: as many waves as fit (SOLHIP_AO_WAVES per SIMD, 4 SIMDs per CU);
+    // the step traversal may hold two more stack entries than trace_ray
+    ProgArgs p = a;
+    p.sv.stack_depth = a.sv.stack_depth + 2;
+    const size_t lds = (size_t)p.sv.stack_depth * sizeof(uint2) * kProgBlock;
+    const uint32_t blocks = a.ncu * 4 * SOLHIP_AO_WAVES;
+    if (timer) timer->begin(s, SOLHIP_KERNEL_AO_RAYS);
+    if (a.ao_any) {
+        if (a.count) hipLaunchKernelGGL((k_ao_rays_persist<true, true>), dim3(blocks), dim3(kProgBlock), lds, s, p);
+        else hipLaunchKernelGGL((k_ao_rays_persist<true, false>), dim3(blocks), dim3(kProgBlock), lds, s, p);
+    } else {
+        if (a.count) hipLaunchKernelGGL((k_ao_rays_persist<false, true>), dim3(blocks), dim3(kProgBlock), lds, s, p);
+        else hipLaunchKernelGGL((k_ao_rays_persist<false, false>), dim3(blocks), dim3(kProgBlock), lds, s, p);
+    }
+    if (timer) timer->end(s, SOLHIP_KERNEL_AO_RAYS);
+    const uint32_t rblocks = (a.W * a.H + kProgBlock - 1) / kProgBlock;
+    hipLaunchKernelGGL(k_ao_resolve, dim3(rblocks), dim3(kProgBlock), 0, s, p);
     return launch_status("AO");
 }
 

@@ -11,6 +11,7 @@
 
 #include "../../sol-rs_amd/csrc/bvh_logic.h"
 #include "../../sol-rs_amd/csrc/traverse.h"
+#include "../../sol-rs_amd/csrc/traverse_step.h"
 
 using namespace solhip;
 
@@ -231,6 +232,53 @@ void hc_trace(void *h, const float *rays, uint32_t n, uint32_t any, void *hits, 
         counts[1] += tc.tris;
         counts[2] += tc.insts;
         counts[3] += tc.overflow ? 1 : 0;
+    }
+}
+
+// the resumable traversal as the persistent AO kernel drives it: 64 lanes,
+// each with its own stack column, take rays from a shared cursor and advance
+// one step per round; a lane that finishes takes the next ray at once
+void hc_trace_steps(void *h, const float *rays, uint32_t n, uint32_t any, void *hits, uint64_t *counts)
+{
+    HostScene *s = static_cast<HostScene *>(h);
+    SceneView sv;
+    sv.tnodes = s->insts.empty() ? nullptr : s->tlas.nodes.data();
+    sv.insts = s->insts.data();
+    sv.inst_meta = s->meta.data();
+    uint32_t max_blas_depth = 0;
+    for (const auto &b : s->blas) max_blas_depth = std::max(max_blas_depth, b.bvh.depth);
+    sv.stack_depth = std::min<uint32_t>(kMaxStack, s->tlas.depth + max_blas_depth + 1) + 2;
+    constexpr int kLanes = 64;
+    std::vector<uint2> lds((size_t)(kMaxStack + 2) * kLanes);
+    TravState st[kLanes];
+    int64_t rid[kLanes];
+    TravCounts tc[kLanes];
+    uint32_t cursor = 0;
+    for (int l = 0; l < kLanes; ++l) rid[l] = -1;
+    while (true) {
+        bool any_live = false;
+        for (int l = 0; l < kLanes; ++l) {
+            if (rid[l] < 0 && cursor < n) {
+                rid[l] = cursor++;
+                const float *r = rays + 8 * (size_t)rid[l];
+                trav_init(st[l], sv, v3(r[0], r[1], r[2]), v3(r[4], r[5], r[6]), r[3], r[7]);
+                tc[l] = TravCounts{0, 0, 0, false};
+            }
+            if (rid[l] < 0) continue;
+            any_live = true;
+            const bool done = any ? trav_step<true, true>(st[l], sv, 0xFFu, lds.data() + l, kLanes, tc[l])
+                                  : trav_step<false, true>(st[l], sv, 0xFFu, lds.data() + l, kLanes, tc[l]);
+            if (done) {
+                const Hit hit = trav_result(st[l]);
+                std::memcpy((char *)hits + 20 * (size_t)rid[l], &hit, 20);
+                counts[0] += tc[l].nodes;
+                counts[1] += tc[l].tris;
+                counts[2] += tc[l].insts;
+                counts[3] += tc[l].overflow ? 1 : 0;
+                rid[l] = -1;
+            }
+        }
+        if (!any_live && cursor >= n) break;
     }
 }
 

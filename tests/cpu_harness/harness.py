@@ -24,6 +24,7 @@ def lib():
         L.hc_scene_destroy.argtypes = [vp]
         L.hc_scene_info.argtypes = [vp, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.hc_trace.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, vp]
+        L.hc_trace_steps.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, vp]
         _lib = L
     return _lib
 
@@ -55,12 +56,14 @@ class HostScene:
         lib().hc_scene_info(self.h, blas, C.byref(a), C.byref(b))
         return a.value, b.value
 
-    def trace(self, rays, any_hit=False):
+    def trace(self, rays, any_hit=False, steps=False):
+        """trace_ray per ray, or (steps) the resumable traversal driven as
+        64 refilling lanes"""
         rays = np.ascontiguousarray(rays, RAY)
         hits = np.zeros(len(rays), HIT)
         counts = np.zeros(4, np.uint64)
-        lib().hc_trace(self.h, rays.ctypes.data, len(rays), 1 if any_hit else 0, hits.ctypes.data,
-                       counts.ctypes.data)
+        fn = lib().hc_trace_steps if steps else lib().hc_trace
+        fn(self.h, rays.ctypes.data, len(rays), 1 if any_hit else 0, hits.ctypes.data, counts.ctypes.data)
         return hits, counts
 
     def __del__(self):

@@ -50,3 +50,19 @@ def test_replay_tree_shape_heightfield():
     hs = harness.HostScene(sd)
     nodes, depth = hs.info(0)
     assert nodes < sd.tri_count // 2 and 3 <= depth <= 16
+
+
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_resumable_traversal_equals_trace_ray(name):
+    """the resumable traversal (traverse_step.h) driven as 64 lanes that take
+    a new ray as soon as theirs finishes gives trace_ray's closest and first
+    hits bit for bit, with the same fetch counts and no stack overflow"""
+    sd = SCENES[name]()
+    hs = harness.HostScene(sd)
+    rays = np.concatenate([raygen.random_rays(3000, (0, 0, 0), 2.0 if name != "instances" else 6.0, seed=21),
+                           raygen.aimed_rays(sd, 1500, seed=22), raygen.axis_rays(500)])
+    for any_hit in (False, True):
+        a, ca = hs.trace(rays, any_hit=any_hit)
+        b, cb = hs.trace(rays, any_hit=any_hit, steps=True)
+        assert np.array_equal(a.view(np.uint32).reshape(-1), b.view(np.uint32).reshape(-1)), (name, any_hit)
+        assert np.array_equal(ca[:3], cb[:3]) and cb[3] == 0
