@@ -259,11 +259,12 @@ int trace_wavefront(Context *c, hipStream_t s, Program *p, Tlas *t, const ProgAr
     if (tiles * 64 > 0xFFFFFFF0ull) return bad_arg("image too large");
     w.nslots = (uint32_t)(tiles * 64);
     w.block = p->spec.wf_block ? p->spec.wf_block : 64u;
-    if ((uint64_t)w.sv.stack_depth * sizeof(uint2) * w.block > 65536) {
+    if ((uint64_t)(w.sv.stack_depth + 2) * sizeof(uint2) * w.block > 65536) {
         set_error("traversal stack of the wavefront kernels exceeds 64 KiB of LDS; use a smaller wf_block");
         return SOLHIP_ERROR_UNSUPPORTED;
     }
-    const uint32_t per_cu = p->spec.wf_blocks_per_cu ? p->spec.wf_blocks_per_cu : std::max(1u, 1024u / w.block);
+    // default: 24 waves per CU (6 per SIMD, what the traversal kernels' registers allow)
+    const uint32_t per_cu = p->spec.wf_blocks_per_cu ? p->spec.wf_blocks_per_cu : std::max(1u, 1536u / w.block);
     w.persist_blocks = c->ncu * per_cu;
     w.shade_blocks = (uint32_t)std::min<uint64_t>((w.nslots + w.block - 1) / w.block, (uint64_t)c->ncu * 2048 / w.block);
     w.counters = c->counters;
@@ -942,8 +943,8 @@ int32_t solhip_pathtrace_autotune(solhip_ctx ctx, solhip_stream st, solhip_tlas 
     int rc = SOLHIP_OK;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = SOLHIP_ERROR_DEVICE;
     for (uint32_t blk : {64u, 128u, 256u}) {
-        if ((uint64_t)depth * 8 * blk > 65536) continue;
-        const uint32_t base = 1024u / blk;
+        if ((uint64_t)(depth + 2) * 8 * blk > 65536) continue;
+        const uint32_t base = 1536u / blk;
         for (uint32_t pcu : {base / 2 ? base / 2 : 1u, base, 2 * base}) {
             for (uint32_t gr : {0u, 1u}) {
                 if (rc) break;

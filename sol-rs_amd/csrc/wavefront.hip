@@ -29,6 +29,7 @@
 #include "pathshade.h"
 #include "shading.h"
 #include "traverse.h"
+#include "traverse_step.h"
 
 namespace solhip {
 
@@ -111,6 +112,44 @@ __global__ void k_pt_generate(WfArgs A, uint32_t pass)
     A.q[0][slot] = r;
 }
 
+// register budget of the traversal kernels: waves per SIMD they must fit
+#ifndef SOLHIP_WF_WAVES
+#define SOLHIP_WF_WAVES 6
+#endif
+#define SOLHIP_WF_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SOLHIP_WF_WAVES)))
+
+// lanes of a wave take new queue slots once at least this many are idle
+// (0: one slot per lane per 64-slot fetch, trace_ray to completion)
+#ifndef SOLHIP_WF_REFILL
+#define SOLHIP_WF_REFILL 16
+#endif
+
+template <bool ANY>
+__device__ __forceinline__ void queue_result(const WfArgs &A, uint32_t i, uint32_t path, const Hit &h)
+{
+    if (!ANY) {
+        solhip_hit out;
+        out.t = h.t;
+        out.u = h.u;
+        out.v = h.v;
+        out.inst = h.inst;
+        out.prim = h.prim;
+        A.hits[i] = out;
+    } else if (h.inst == SOLHIP_MISS) {
+        const float4 c = A.scontrib[i];
+        float4 s1 = A.st1[path];
+        s1.x = s1.x + c.x;
+        s1.y = s1.y + c.y;
+        s1.z = s1.z + c.z;
+        A.st1[path] = s1;
+    }
+}
+
+// Extension (closest hit -> hits[i]) or shadow (any hit; unoccluded ->
+// contribution into the path radiance) for queue slots [0, cnt), taken from
+// the atomic head.  The resumable traversal lets a lane start its next slot
+// while the rest of the wave is still traversing.  sv.stack_depth includes
+// the two extra entries the step traversal may hold.
 template <bool ANY, bool COUNT>
 __device__ __forceinline__ void trace_queue(const WfArgs &A, const QRay *q, uint32_t cnt, uint32_t *head, int kernel)
 {
@@ -119,6 +158,47 @@ __device__ __forceinline__ void trace_queue(const WfArgs &A, const QRay *q, uint
     const float tmin = ray_tmin(A);
     TravCounts tc{0, 0, 0, false};
     uint32_t rays = 0;
+#if SOLHIP_WF_REFILL
+    TravState st;
+    bool has = false, exhausted = false;
+    uint32_t idx = 0, path = 0;
+    while (true) {
+        const uint64_t idle = __ballot(!has);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (!exhausted && nidle >= (uint32_t)SOLHIP_WF_REFILL) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(head, nidle);
+            base = __shfl(base, 0);
+            exhausted = base + nidle >= cnt;
+            if (!has) {
+                const uint32_t my = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                if (my < cnt) {
+                    const QRay r = q[my];
+                    if (r.path == kNoPath) {
+                        if (!ANY) {
+                            Hit miss;
+                            miss.t = -1.0f;
+                            miss.u = miss.v = 0.0f;
+                            miss.inst = miss.prim = SOLHIP_MISS;
+                            queue_result<false>(A, my, r.path, miss);
+                        }
+                    } else {
+                        trav_init(st, A.sv, v3(r.ox, r.oy, r.oz), v3(r.dx, r.dy, r.dz), tmin, r.tmax);
+                        idx = my;
+                        path = r.path;
+                        has = true;
+                    }
+                }
+            }
+        }
+        if (__ballot(has) == 0) break;
+        if (has && trav_step<ANY, COUNT>(st, A.sv, 0xFFu, stack + threadIdx.x, A.block, tc)) {
+            queue_result<ANY>(A, idx, path, trav_result(st));
+            ++rays;
+            has = false;
+        }
+    }
+#else
     for (;;) {
         uint32_t base = 0;
         if (lane == 0) base = atomicAdd(head, 64u);
@@ -127,42 +207,24 @@ __device__ __forceinline__ void trace_queue(const WfArgs &A, const QRay *q, uint
         const uint32_t i = base + lane;
         if (i < cnt) {
             const QRay r = q[i];
-            if (!ANY) {
-                solhip_hit out;
-                out.t = -1.0f;
-                out.u = out.v = 0.0f;
-                out.inst = out.prim = SOLHIP_MISS;
-                if (r.path != kNoPath) {
-                    const Hit h = trace_ray<false, COUNT>(A.sv, v3(r.ox, r.oy, r.oz), v3(r.dx, r.dy, r.dz), tmin,
-                                                          r.tmax, 0xFFu, stack + threadIdx.x, A.block, tc);
-                    ++rays;
-                    out.t = h.t;
-                    out.u = h.u;
-                    out.v = h.v;
-                    out.inst = h.inst;
-                    out.prim = h.prim;
-                }
-                A.hits[i] = out;
-            } else {
-                const Hit h = trace_ray<true, COUNT>(A.sv, v3(r.ox, r.oy, r.oz), v3(r.dx, r.dy, r.dz), tmin, r.tmax,
-                                                     0xFFu, stack + threadIdx.x, A.block, tc);
+            Hit h;
+            h.t = -1.0f;
+            h.u = h.v = 0.0f;
+            h.inst = h.prim = SOLHIP_MISS;
+            if (r.path != kNoPath) {
+                h = trace_ray<ANY, COUNT>(A.sv, v3(r.ox, r.oy, r.oz), v3(r.dx, r.dy, r.dz), tmin, r.tmax, 0xFFu,
+                                          stack + threadIdx.x, A.block, tc);
                 ++rays;
-                if (h.inst == SOLHIP_MISS) {
-                    const float4 c = A.scontrib[i];
-                    float4 s1 = A.st1[r.path];
-                    s1.x = s1.x + c.x;
-                    s1.y = s1.y + c.y;
-                    s1.z = s1.z + c.z;
-                    A.st1[r.path] = s1;
-                }
             }
+            if (!ANY || r.path != kNoPath) queue_result<ANY>(A, i, r.path, h);
         }
     }
+#endif
     count_rays<COUNT>(A.counters + kernel, rays, tc);
 }
 
 template <bool COUNT>
-__global__ void k_pt_extend(WfArgs A, uint32_t b)
+__global__ void SOLHIP_WF_OCCUPANCY k_pt_extend(WfArgs A, uint32_t b)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         // the queues this bounce's shade fills were last read a bounce ago
@@ -173,7 +235,7 @@ __global__ void k_pt_extend(WfArgs A, uint32_t b)
 }
 
 template <bool COUNT>
-__global__ void k_pt_shadow(WfArgs A, uint32_t b)
+__global__ void SOLHIP_WF_OCCUPANCY k_pt_shadow(WfArgs A, uint32_t b)
 {
     trace_queue<true, COUNT>(A, A.sq, A.counts[2], A.heads + 2 * b + 1, SOLHIP_KERNEL_PT_SHADOW);
 }
@@ -575,8 +637,11 @@ int launch_set_frame(hipStream_t s, FrameConsts *dst, const FrameConsts &value)
     return launch_status("frame constants");
 }
 
-int launch_pathtrace(hipStream_t s, const WfArgs &a, Timer *timer, const PassHook *hook)
+int launch_pathtrace(hipStream_t s, const WfArgs &a0, Timer *timer, const PassHook *hook)
 {
+    // the step traversal may hold two more stack entries than trace_ray
+    WfArgs a = a0;
+    if (SOLHIP_WF_REFILL) a.sv.stack_depth += 2;
     const uint32_t blk = a.block;
     const size_t lds = (size_t)a.sv.stack_depth * sizeof(uint2) * blk;
     const uint32_t nheads = 2 * (a.max_depth + 1);
