@@ -30,6 +30,7 @@
 #include "pathshade.h"
 #include "shading.h"
 #include "traverse.h"
+#include "traverse_step.h"
 
 namespace solhip {
 
@@ -143,6 +144,9 @@ __global__ void k_srv_admit(SrvArgs A, uint32_t j, uint32_t cur, uint32_t nslots
     if (valid) A.q[cur][at] = r;
 }
 
+// extension / shadow rays of all running jobs, each against its job's scene
+// through the resumable traversal; a lane takes new slots once at least 16
+// lanes of its wave are idle
 template <bool ANY>
 __device__ __forceinline__ void srv_trace(const SrvArgs &A, const QRay *q, uint32_t cnt, uint32_t *head, int kernel)
 {
@@ -150,17 +154,35 @@ __device__ __forceinline__ void srv_trace(const SrvArgs &A, const QRay *q, uint3
     const uint32_t lane = threadIdx.x & 63u;
     TravCounts tc{0, 0, 0, false};
     uint32_t rays = 0;
-    for (;;) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(head, 64u);
-        base = __shfl(base, 0);
-        if (base >= cnt) break;
-        const uint32_t i = base + lane;
-        if (i < cnt) {
-            const QRay r = q[i];
-            const JobDev &J = A.jobs[A.pmeta[r.path] & 0xFFu];
-            const Hit h = trace_ray<ANY, false>(J.sv, v3(r.ox, r.oy, r.oz), v3(r.dx, r.dy, r.dz), job_tmin(J), r.tmax,
-                                                0xFFu, stack + threadIdx.x, kSrvBlock, tc);
+    TravState st;
+    SceneView sv;
+    bool has = false, exhausted = false;
+    uint32_t idx = 0, path = 0;
+    while (true) {
+        const uint64_t idle = __ballot(!has);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (!exhausted && nidle >= 16u) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(head, nidle);
+            base = __shfl(base, 0);
+            exhausted = base + nidle >= cnt;
+            if (!has) {
+                const uint32_t my = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                if (my < cnt) {
+                    const QRay r = q[my];
+                    const JobDev &J = A.jobs[A.pmeta[r.path] & 0xFFu];
+                    sv = J.sv;
+                    sv.stack_depth += 2;   // the step traversal's two extra entries
+                    trav_init(st, sv, v3(r.ox, r.oy, r.oz), v3(r.dx, r.dy, r.dz), job_tmin(J), r.tmax);
+                    idx = my;
+                    path = r.path;
+                    has = true;
+                }
+            }
+        }
+        if (__ballot(has) == 0) break;
+        if (has && trav_step<ANY, false>(st, sv, 0xFFu, stack + threadIdx.x, kSrvBlock, tc)) {
+            const Hit h = trav_result(st);
             ++rays;
             if (!ANY) {
                 solhip_hit out;
@@ -169,21 +191,23 @@ __device__ __forceinline__ void srv_trace(const SrvArgs &A, const QRay *q, uint3
                 out.v = h.v;
                 out.inst = h.inst;
                 out.prim = h.prim;
-                A.hits[i] = out;
+                A.hits[idx] = out;
             } else if (h.inst == SOLHIP_MISS) {
-                const float4 c = A.scontrib[i];
-                float4 s1 = A.st1[r.path];
+                const float4 c = A.scontrib[idx];
+                float4 s1 = A.st1[path];
                 s1.x = s1.x + c.x;
                 s1.y = s1.y + c.y;
                 s1.z = s1.z + c.z;
-                A.st1[r.path] = s1;
+                A.st1[path] = s1;
             }
+            has = false;
         }
     }
     count_rays<false>(A.counters + kernel, rays, tc);
 }
 
-__global__ void __launch_bounds__(kSrvBlock) k_srv_extend(SrvArgs A, uint32_t cur)
+__global__ void __launch_bounds__(kSrvBlock) __attribute__((amdgpu_waves_per_eu(6))) k_srv_extend(SrvArgs A,
+                                                                                                  uint32_t cur)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         // filled by this step's shade / retire; last read a step ago
@@ -194,7 +218,7 @@ __global__ void __launch_bounds__(kSrvBlock) k_srv_extend(SrvArgs A, uint32_t cu
     srv_trace<false>(A, A.q[cur], A.counts[cur], A.heads + 0, SOLHIP_KERNEL_SRV_EXTEND);
 }
 
-__global__ void __launch_bounds__(kSrvBlock) k_srv_shadow(SrvArgs A)
+__global__ void __launch_bounds__(kSrvBlock) __attribute__((amdgpu_waves_per_eu(6))) k_srv_shadow(SrvArgs A)
 {
     srv_trace<true>(A, A.sq, A.counts[2], A.heads + 1, SOLHIP_KERNEL_SRV_SHADOW);
 }
@@ -479,9 +503,9 @@ struct Server : Obj {
     int step()
     {
         const SrvArgs a = args();
-        const uint32_t persist = cx->ncu * 16;
+        const uint32_t persist = cx->ncu * 24;
         const uint32_t shade_blocks = std::min<uint32_t>((max_slots + kSrvBlock - 1) / kSrvBlock, cx->ncu * 32);
-        const size_t lds = (size_t)max_stack * sizeof(uint2) * kSrvBlock;
+        const size_t lds = (size_t)(max_stack + 2) * sizeof(uint2) * kSrvBlock;
         Timer *timer = cx->timer_or_null();
         auto timed = [&](int id, auto &&launch) {
             if (timer) timer->begin(stream, id);
