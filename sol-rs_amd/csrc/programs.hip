@@ -23,13 +23,13 @@ namespace solhip {
 // minimum waves per SIMD requested for the AO ray kernel: it is VALU-issue
 // bound, so the register budget is traded against latency hiding
 #ifndef SOLHIP_AO_WAVES
-#define SOLHIP_AO_WAVES 6
+#define SOLHIP_AO_WAVES 5
 #endif
 #define SOLHIP_AO_RAYS_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SOLHIP_AO_WAVES)))
 // persistent AO rays: a wave takes new rays for its idle lanes once at least
 // this many are idle (0: one ray per lane, no refill)
 #ifndef SOLHIP_AO_REFILL
-#define SOLHIP_AO_REFILL 16
+#define SOLHIP_AO_REFILL 32
 #endif
 
 // ---------------------------------------------------------------------------
