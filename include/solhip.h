@@ -177,7 +177,8 @@ typedef struct {
      * (training reads its batch size back). */
     uint32_t nrc;             /* 1: use the cache */
     uint32_t nrc_train_steps; /* Adam steps per frame (0 = 4) */
-    uint32_t reserved[2];
+    float nrc_learning_rate;  /* Adam step size (0 = 1e-3) */
+    uint32_t reserved[1];
 } solhip_spec;
 
 /* per-dispatch images: W*H pixels, row-major, no padding */

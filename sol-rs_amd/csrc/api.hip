@@ -184,7 +184,8 @@ static int nrc_query_hook(void *user, hipStream_t s)
 int trace_nrc(Context *c, hipStream_t s, Program *p, Tlas *t, WfArgs &w, Timer *timer)
 {
     if (!p->nrc_mlp) {
-        p->nrc_mlp = mlp_new(c, 1u, 1e-3f);
+        const float lr = p->spec.nrc_learning_rate > 0.0f ? p->spec.nrc_learning_rate : 1e-3f;
+        p->nrc_mlp = mlp_new(c, 1u, lr);
         if (!p->nrc_mlp) return SOLHIP_ERROR_DEVICE;
     }
     NrcArgs &N = w.nrc;

@@ -29,7 +29,7 @@ namespace solhip {
 // persistent AO rays: a wave takes new rays for its idle lanes once at least
 // this many are idle (0: one ray per lane, no refill)
 #ifndef SOLHIP_AO_REFILL
-#define SOLHIP_AO_REFILL 32
+#define SOLHIP_AO_REFILL 40
 #endif
 
 // ---------------------------------------------------------------------------
