@@ -12,7 +12,9 @@
 //      histograms, one exclusive scan, per-tile stable scatter whose local
 //      ranks come from wave64 ballots (match on the 8 digit bits)
 //   4. Karras 2012 binary radix tree (index tie-break for equal codes)
-//   5. bottom-up box refit with one agent-scope acq_rel counter per node
+//   5. bottom-up box refit in two phases: inside each 256-leaf block with
+//      LDS flags, then across blocks from the frontier nodes with one
+//      agent-scope acq_rel flag per node
 //   6. top-down collapse, one launch per level, to 8-wide nodes: a node
 //      repeatedly opens its largest-area internal child until it has 8
 //      children; children of <= leaf_max primitives become leaves, the others
