@@ -265,7 +265,7 @@ int trace_wavefront(Context *c, hipStream_t s, Program *p, Tlas *t, const ProgAr
         return SOLHIP_ERROR_UNSUPPORTED;
     }
     // default: 24 waves per CU (6 per SIMD, what the traversal kernels' registers allow)
-    const uint32_t per_cu = p->spec.wf_blocks_per_cu ? p->spec.wf_blocks_per_cu : std::max(1u, 1536u / w.block);
+    const uint32_t per_cu = p->spec.wf_blocks_per_cu ? p->spec.wf_blocks_per_cu : std::max(1u, wf_threads_per_cu() / w.block);
     w.persist_blocks = c->ncu * per_cu;
     w.shade_blocks = (uint32_t)std::min<uint64_t>((w.nslots + w.block - 1) / w.block, (uint64_t)c->ncu * 2048 / w.block);
     w.counters = c->counters;
@@ -945,7 +945,7 @@ int32_t solhip_pathtrace_autotune(solhip_ctx ctx, solhip_stream st, solhip_tlas 
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = SOLHIP_ERROR_DEVICE;
     for (uint32_t blk : {64u, 128u, 256u}) {
         if ((uint64_t)(depth + 2) * 8 * blk > 65536) continue;
-        const uint32_t base = 1536u / blk;
+        const uint32_t base = std::max(1u, wf_threads_per_cu() / blk);
         for (uint32_t pcu : {base / 2 ? base / 2 : 1u, base, 2 * base}) {
             for (uint32_t gr : {0u, 1u}) {
                 if (rc) break;

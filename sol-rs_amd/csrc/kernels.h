@@ -169,6 +169,8 @@ struct PassHook {
 // one frame of the wavefront path tracer (kernels only: graph-capturable
 // when the radiance cache is off)
 int launch_pathtrace(hipStream_t s, const WfArgs &a, Timer *timer, const PassHook *hook = nullptr);
+// threads per CU that the traversal kernels' register budget keeps resident
+uint32_t wf_threads_per_cu();
 // writes the frame constants through a kernel argument (no host buffer to keep alive)
 int launch_set_frame(hipStream_t s, FrameConsts *dst, const FrameConsts &value);
 

@@ -37,6 +37,12 @@ namespace solhip {
 namespace {
 
 constexpr uint32_t kSrvBlock = 64;
+
+// register budget of the traversal kernels: waves per SIMD they must fit
+#ifndef SOLHIP_SRV_WAVES
+#define SOLHIP_SRV_WAVES 5
+#endif
+#define SOLHIP_SRV_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SOLHIP_SRV_WAVES)))
 constexpr uint32_t kMaxJobs = 256;   // job index lives in 8 bits of the slot meta word
 
 // per-job constants (device array, one entry per running-job index)
@@ -206,7 +212,7 @@ __device__ __forceinline__ void srv_trace(const SrvArgs &A, const QRay *q, uint3
     count_rays<false>(A.counters + kernel, rays, tc);
 }
 
-__global__ void __launch_bounds__(kSrvBlock) __attribute__((amdgpu_waves_per_eu(6))) k_srv_extend(SrvArgs A,
+__global__ void __launch_bounds__(kSrvBlock) SOLHIP_SRV_OCCUPANCY k_srv_extend(SrvArgs A,
                                                                                                   uint32_t cur)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -218,7 +224,7 @@ __global__ void __launch_bounds__(kSrvBlock) __attribute__((amdgpu_waves_per_eu(
     srv_trace<false>(A, A.q[cur], A.counts[cur], A.heads + 0, SOLHIP_KERNEL_SRV_EXTEND);
 }
 
-__global__ void __launch_bounds__(kSrvBlock) __attribute__((amdgpu_waves_per_eu(6))) k_srv_shadow(SrvArgs A)
+__global__ void __launch_bounds__(kSrvBlock) SOLHIP_SRV_OCCUPANCY k_srv_shadow(SrvArgs A)
 {
     srv_trace<true>(A, A.sq, A.counts[2], A.heads + 1, SOLHIP_KERNEL_SRV_SHADOW);
 }
@@ -503,7 +509,7 @@ struct Server : Obj {
     int step()
     {
         const SrvArgs a = args();
-        const uint32_t persist = cx->ncu * 24;
+        const uint32_t persist = cx->ncu * 4 * SOLHIP_SRV_WAVES;
         const uint32_t shade_blocks = std::min<uint32_t>((max_slots + kSrvBlock - 1) / kSrvBlock, cx->ncu * 32);
         const size_t lds = (size_t)(max_stack + 2) * sizeof(uint2) * kSrvBlock;
         Timer *timer = cx->timer_or_null();

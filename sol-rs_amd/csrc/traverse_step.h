@@ -1,6 +1,7 @@
 // traverse_step.h -- the two-level traversal of traverse.h as a resumable
-// state machine: trav_step does one unit of work (one triangle, one node, one
-// instance entry or exit, or one stack pop) and reports when the ray is done.
+// state machine: trav_step does one step of work (at most one triangle, one
+// instance entry and one node, with the stack pops between them) and reports
+// when the ray is done.
 // A lane whose ray has finished can then take the next ray while the rest of
 // its wave keeps traversing (persistent threads with per-lane refill), which
 // is what incoherent rays (AO, bounces) need to keep the SIMD busy.
@@ -73,81 +74,112 @@ SOLHIP_HD Hit trav_result(const TravState &s)
     return h;
 }
 
-// One unit of work; true when the ray is finished.  lds: this lane's stack
+// One traversal step; true when the ray is finished.  A step does, in this
+// order and each at most once: one triangle of the current leaf group, the
+// stack pops and BLAS exits that lead to the next piece of work, one instance
+// entry, one node test.  Running them as phases of one step (rather than one
+// of them per step) lets a wave's lanes share the expensive node test in the
+// same pass whatever bookkeeping each needed first.  lds: this lane's stack
 // column, stride = lanes sharing the array, sv.stack_depth entries.
 template <bool ANY, bool COUNT>
 SOLHIP_HD bool trav_step(TravState &s, const SceneView &sv, uint32_t cull, uint2 *lds, int stride, TravCounts &cnt)
 {
     const int depth = (int)sv.stack_depth;
-    if (s.tg_mask) {
+    // 1. one triangle
+    if (s.level && s.tg_mask) {
         const uint32_t k = __builtin_ctz(s.tg_mask);
         s.tg_mask &= s.tg_mask - 1u;
-        if (s.level) {
-            // one triangle
-            const uint4 *tp = reinterpret_cast<const uint4 *>(s.tris + s.tg_base + k);
-            const uint4 a = SOLHIP_G4(tp)[0], b = SOLHIP_G4(tp)[1];
-            const uint2 c = SOLHIP_G2(tp + 2)[0];
-            const float vv[9] = {as_f(a.x), as_f(a.y), as_f(a.z), as_f(a.w), as_f(b.x),
-                                 as_f(b.y), as_f(b.z), as_f(b.w), as_f(c.x)};
-            if (COUNT) cnt.tris++;
-            float t, u, v;
-            if (wt_hit(s.wt, s.tmin, s.best.t, vv, t, u, v) && better(t, s.inst, c.y, s.best)) {
-                s.best.t = t;
-                s.best.u = u;
-                s.best.v = v;
-                s.best.inst = s.inst;
-                s.best.prim = c.y;
-                if (ANY) return true;
-            }
-            return false;
+        const uint4 *tp = reinterpret_cast<const uint4 *>(s.tris + s.tg_base + k);
+        const uint4 a = SOLHIP_G4(tp)[0], b = SOLHIP_G4(tp)[1];
+        const uint2 c = SOLHIP_G2(tp + 2)[0];
+        const float vv[9] = {as_f(a.x), as_f(a.y), as_f(a.z), as_f(a.w), as_f(b.x),
+                             as_f(b.y), as_f(b.z), as_f(b.w), as_f(c.x)};
+        if (COUNT) cnt.tris++;
+        float t, u, v;
+        if (wt_hit(s.wt, s.tmin, s.best.t, vv, t, u, v) && better(t, s.inst, c.y, s.best)) {
+            s.best.t = t;
+            s.best.u = u;
+            s.best.v = v;
+            s.best.inst = s.inst;
+            s.best.prim = c.y;
+            if (ANY) return true;
         }
-        // an instance leaf: enter its BLAS
+    }
+    // 2. pops and BLAS exits up to the next node group or leaf group
+    while (!s.tg_mask && !s.ng_hits) {
+        if (s.level && s.sp == s.base) {
+            // the BLAS is done: back to the world ray and the TLAS stack
+            s.level = 0;
+            s.o = s.wo;
+            s.inv = safe_inv(s.wd);
+            s.oct = octant(s.wd);
+        } else if (s.sp > 0) {
+            --s.sp;
+            const uint2 e = lds[s.sp * stride];
+            if (e.y & kLeafGroupTag) {
+                s.tg_base = e.x;
+                s.tg_mask = e.y & ~kLeafGroupTag;
+            } else {
+                s.ng_base = e.x;
+                s.ng_hits = e.y & 0xFFu;
+                s.ng_imask = (e.y >> 8) & 0xFFu;
+            }
+        } else {
+            return true;
+        }
+    }
+    // 3. an instance leaf: enter its BLAS
+    if (!s.level && s.tg_mask) {
+        const uint32_t k = __builtin_ctz(s.tg_mask);
+        s.tg_mask &= s.tg_mask - 1u;
         const uint32_t li = s.tg_base + k;
         const uint32_t meta = SOLHIP_G1(sv.inst_meta + li)[0];
-        if (((meta >> 24) & cull) == 0u) return false;
-        const uint4 *ip = reinterpret_cast<const uint4 *>(sv.insts + li);
-        const uint4 i0 = SOLHIP_G4(ip)[0], i1 = SOLHIP_G4(ip)[1], i2 = SOLHIP_G4(ip)[2], i3 = SOLHIP_G4(ip)[3];
-        if (COUNT) cnt.insts++;
-        const uint64_t bn = ((uint64_t)i3.y << 32) | i3.x;
-        if (bn == 0) return false;
-        // the TLAS continues later: pending node group, then pending leaves
-        if (s.ng_hits) {
-            if (s.sp < depth) {
-                lds[s.sp * stride] = make_uint2(s.ng_base, s.ng_hits | (s.ng_imask << 8));
-                ++s.sp;
-            } else {
-                cnt.overflow = true;
+        if (((meta >> 24) & cull) != 0u) {
+            const uint4 *ip = reinterpret_cast<const uint4 *>(sv.insts + li);
+            const uint4 i0 = SOLHIP_G4(ip)[0], i1 = SOLHIP_G4(ip)[1], i2 = SOLHIP_G4(ip)[2], i3 = SOLHIP_G4(ip)[3];
+            if (COUNT) cnt.insts++;
+            const uint64_t bn = ((uint64_t)i3.y << 32) | i3.x;
+            if (bn != 0) {
+                // the TLAS continues later: pending node group, then pending leaves
+                if (s.ng_hits) {
+                    if (s.sp < depth) {
+                        lds[s.sp * stride] = make_uint2(s.ng_base, s.ng_hits | (s.ng_imask << 8));
+                        ++s.sp;
+                    } else {
+                        cnt.overflow = true;
+                    }
+                }
+                if (s.tg_mask) {
+                    if (s.sp < depth) {
+                        lds[s.sp * stride] = make_uint2(s.tg_base, s.tg_mask | kLeafGroupTag);
+                        ++s.sp;
+                    } else {
+                        cnt.overflow = true;
+                    }
+                }
+                const float m[12] = {as_f(i0.x), as_f(i0.y), as_f(i0.z), as_f(i0.w), as_f(i1.x), as_f(i1.y),
+                                     as_f(i1.z), as_f(i1.w), as_f(i2.x), as_f(i2.y), as_f(i2.z), as_f(i2.w)};
+                const V3 o = xform_point(m, s.wo);
+                const V3 d = xform_dir(m, s.wd);
+                s.o = o;
+                s.inv = safe_inv(d);
+                s.oct = octant(d);
+                s.wt = wt_setup(o, d);
+                s.nodes = reinterpret_cast<const WideNode *>(bn);
+                s.tris = reinterpret_cast<const TriRec *>(((uint64_t)i3.w << 32) | i3.z);
+                s.inst = meta & 0xFFFFFFu;
+                s.level = 1;
+                s.base = s.sp;
+                s.ng_base = 0;
+                s.ng_hits = 1u << s.oct;
+                s.ng_imask = 1u;
+                s.tg_base = 0;
+                s.tg_mask = 0;
             }
         }
-        if (s.tg_mask) {
-            if (s.sp < depth) {
-                lds[s.sp * stride] = make_uint2(s.tg_base, s.tg_mask | kLeafGroupTag);
-                ++s.sp;
-            } else {
-                cnt.overflow = true;
-            }
-        }
-        const float m[12] = {as_f(i0.x), as_f(i0.y), as_f(i0.z), as_f(i0.w), as_f(i1.x), as_f(i1.y),
-                             as_f(i1.z), as_f(i1.w), as_f(i2.x), as_f(i2.y), as_f(i2.z), as_f(i2.w)};
-        const V3 o = xform_point(m, s.wo);
-        const V3 d = xform_dir(m, s.wd);
-        s.o = o;
-        s.inv = safe_inv(d);
-        s.oct = octant(d);
-        s.wt = wt_setup(o, d);
-        s.nodes = reinterpret_cast<const WideNode *>(bn);
-        s.tris = reinterpret_cast<const TriRec *>(((uint64_t)i3.w << 32) | i3.z);
-        s.inst = meta & 0xFFFFFFu;
-        s.level = 1;
-        s.base = s.sp;
-        s.ng_base = 0;
-        s.ng_hits = 1u << s.oct;
-        s.ng_imask = 1u;
-        s.tg_base = 0;
-        s.tg_mask = 0;
-        return false;
     }
-    if (s.ng_hits) {
+    // 4. one node
+    if (!s.tg_mask && s.ng_hits) {
         const uint32_t i = __builtin_ctz(s.ng_hits);
         s.ng_hits &= s.ng_hits - 1u;
         const uint32_t slot = i ^ s.oct;
@@ -170,30 +202,8 @@ SOLHIP_HD bool trav_step(TravState &s, const SceneView &sv, uint32_t cull, uint2
         s.ng_base = w1.x;
         s.ng_imask = w0.w >> 24;
         s.tg_base = w1.y;
-        return false;
     }
-    if (s.level && s.sp == s.base) {
-        // the BLAS is done: back to the world ray and the TLAS stack
-        s.level = 0;
-        s.o = s.wo;
-        s.inv = safe_inv(s.wd);
-        s.oct = octant(s.wd);
-        return false;
-    }
-    if (s.sp > 0) {
-        --s.sp;
-        const uint2 e = lds[s.sp * stride];
-        if (e.y & kLeafGroupTag) {
-            s.tg_base = e.x;
-            s.tg_mask = e.y & ~kLeafGroupTag;
-        } else {
-            s.ng_base = e.x;
-            s.ng_hits = e.y & 0xFFu;
-            s.ng_imask = (e.y >> 8) & 0xFFu;
-        }
-        return false;
-    }
-    return true;
+    return false;
 }
 
 }  // namespace solhip

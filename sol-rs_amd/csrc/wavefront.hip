@@ -114,7 +114,7 @@ __global__ void k_pt_generate(WfArgs A, uint32_t pass)
 
 // register budget of the traversal kernels: waves per SIMD they must fit
 #ifndef SOLHIP_WF_WAVES
-#define SOLHIP_WF_WAVES 6
+#define SOLHIP_WF_WAVES 5
 #endif
 #define SOLHIP_WF_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SOLHIP_WF_WAVES)))
 
@@ -608,6 +608,8 @@ int launch_status(const char *what)
 }
 
 }  // namespace
+
+uint32_t wf_threads_per_cu() { return 4u * SOLHIP_WF_WAVES * 64u; }
 
 int launch_lights_count(hipStream_t s, const DevTables &tb, const uint32_t *inst_tris, uint32_t n_inst,
                         uint32_t *offsets, Timer *timer)
