@@ -178,7 +178,12 @@ typedef struct {
     uint32_t nrc;             /* 1: use the cache */
     uint32_t nrc_train_steps; /* Adam steps per frame (0 = 4) */
     float nrc_learning_rate;  /* Adam step size (0 = 1e-3) */
-    uint32_t reserved[1];
+    /* SOLHIP_PROG_PATHTRACE: the image's 8x8 tiles split into this many
+     * contiguous slices whose passes run on their own streams at the same
+     * time, so one slice's slowest rays overlap the others' work; pixels are
+     * independent, so the image does not depend on it.  0 = default (1);
+     * always 1 with SOLHIP_PROFILE_TIME or nrc. */
+    uint32_t wf_slices;
 } solhip_spec;
 
 /* per-dispatch images: W*H pixels, row-major, no padding */

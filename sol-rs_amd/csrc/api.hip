@@ -240,6 +240,9 @@ int trace_nrc(Context *c, hipStream_t s, Program *p, Tlas *t, WfArgs &w, Timer *
 }
 
 // SOLHIP_PROG_PATHTRACE: buffers, emitter table, the wavefront passes
+// image slices of a frame when the spec leaves it open
+constexpr uint32_t kDefaultSlices = 1;
+
 int trace_wavefront(Context *c, hipStream_t s, Program *p, Tlas *t, const ProgArgs &a)
 {
     WfArgs w;
@@ -264,7 +267,7 @@ int trace_wavefront(Context *c, hipStream_t s, Program *p, Tlas *t, const ProgAr
         set_error("traversal stack of the wavefront kernels exceeds 64 KiB of LDS; use a smaller wf_block");
         return SOLHIP_ERROR_UNSUPPORTED;
     }
-    // default: 24 waves per CU (6 per SIMD, what the traversal kernels' registers allow)
+    // default: the waves per CU that the traversal kernels' registers allow
     const uint32_t per_cu = p->spec.wf_blocks_per_cu ? p->spec.wf_blocks_per_cu : std::max(1u, wf_threads_per_cu() / w.block);
     w.persist_blocks = c->ncu * per_cu;
     w.shade_blocks = (uint32_t)std::min<uint64_t>((w.nslots + w.block - 1) / w.block, (uint64_t)c->ncu * 2048 / w.block);
@@ -272,10 +275,16 @@ int trace_wavefront(Context *c, hipStream_t s, Program *p, Tlas *t, const ProgAr
     w.count = a.count;
     const uint64_t npix = (uint64_t)a.W * a.H, ns = w.nslots;
     const uint64_t nheads = 2ull * (w.max_depth + 1);
+    // image slices: profiling times every launch on one stream, and the
+    // radiance cache works on the whole image between passes
+    const uint32_t tiles32 = (uint32_t)tiles;
+    uint32_t nslices = p->spec.wf_slices ? p->spec.wf_slices : kDefaultSlices;
+    if (p->spec.nrc || c->timer_or_null()) nslices = 1;
+    nslices = std::max(1u, std::min(nslices, tiles32));
     if (p->st0.reserve(npix * 16) || p->st1.reserve(npix * 16) || p->psum.reserve(npix * 16) ||
         p->q0.reserve(ns * sizeof(QRay)) || p->q1.reserve(ns * sizeof(QRay)) || p->hits.reserve(ns * sizeof(solhip_hit)) ||
-        p->sq.reserve(ns * sizeof(QRay)) || p->scontrib.reserve(ns * 16) || p->counts.reserve(256) ||
-        p->heads.reserve(nheads * 4) || p->fc.reserve(sizeof(FrameConsts)))
+        p->sq.reserve(ns * sizeof(QRay)) || p->scontrib.reserve(ns * 16) || p->counts.reserve(256ull * nslices) ||
+        p->heads.reserve(nheads * 4 * nslices) || p->fc.reserve(sizeof(FrameConsts)))
         return SOLHIP_ERROR_OUT_OF_MEMORY;
     FrameConsts fc;
     std::memset(&fc, 0, sizeof(fc));
@@ -304,14 +313,50 @@ int trace_wavefront(Context *c, hipStream_t s, Program *p, Tlas *t, const ProgAr
         w.nlights = nl;
     }
     if (p->spec.nrc) return trace_nrc(c, s, p, t, w, timer);
-    if (!p->spec.wf_graph || timer) return launch_pathtrace(s, w, timer);
-    // replay the captured passes; recapture when anything the kernels were
+    if (nslices == 1 && (!p->spec.wf_graph || timer)) return launch_pathtrace(s, w, timer);
+    // the frame on `main`: fork to the slices' streams, their passes, join,
+    // then the image write over the whole image
+    auto frame = [&](hipStream_t main) -> int {
+        if (nslices == 1) return launch_pathtrace(main, w, nullptr);
+        for (uint32_t j = 0; j < nslices; ++j) {
+            if (!p->slice_stream[j]) SOLHIP_HIP_CHECK(hipStreamCreateWithFlags(&p->slice_stream[j], hipStreamNonBlocking));
+            if (!p->join_ev[j]) SOLHIP_HIP_CHECK(hipEventCreateWithFlags(&p->join_ev[j], hipEventDisableTiming));
+        }
+        if (!p->fork_ev) SOLHIP_HIP_CHECK(hipEventCreateWithFlags(&p->fork_ev, hipEventDisableTiming));
+        SOLHIP_HIP_CHECK(hipEventRecord(p->fork_ev, main));
+        for (uint32_t j = 0; j < nslices; ++j) {
+            // slice j: tiles [tiles*j/K, tiles*(j+1)/K), its own counters and
+            // fetch heads, its part of the queue arrays
+            const uint32_t t0 = (uint32_t)((uint64_t)tiles32 * j / nslices);
+            const uint32_t t1 = (uint32_t)((uint64_t)tiles32 * (j + 1) / nslices);
+            WfArgs ws = w;
+            ws.slot_base = t0 * 64u;
+            ws.nslots = (t1 - t0) * 64u;
+            ws.q[0] = w.q[0] + ws.slot_base;
+            ws.q[1] = w.q[1] + ws.slot_base;
+            ws.hits = w.hits + ws.slot_base;
+            ws.sq = w.sq + ws.slot_base;
+            ws.scontrib = w.scontrib + ws.slot_base;
+            ws.counts = w.counts + 64u * j;
+            ws.heads = w.heads + nheads * j;
+            ws.shade_blocks = (uint32_t)std::min<uint64_t>((ws.nslots + w.block - 1) / w.block, w.shade_blocks);
+            hipStream_t ss = p->slice_stream[j];
+            SOLHIP_HIP_CHECK(hipStreamWaitEvent(ss, p->fork_ev, 0));
+            const int rc = launch_pathtrace_passes(ss, ws, nullptr);
+            SOLHIP_HIP_CHECK(hipEventRecord(p->join_ev[j], ss));
+            SOLHIP_HIP_CHECK(hipStreamWaitEvent(main, p->join_ev[j], 0));
+            if (rc) return rc;
+        }
+        return launch_pathtrace_finalize(main, w, nullptr);
+    };
+    if (!p->spec.wf_graph) return frame(s);
+    // replay the captured frame; recapture when anything the kernels were
     // launched with differs (buffers, scene pointers, sizes, spec)
     if (!p->gvalid || std::memcmp(&p->gkey, &w, sizeof(w)) != 0) {
         p->drop_graph();
         if (!p->cap_stream) SOLHIP_HIP_CHECK(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
         SOLHIP_HIP_CHECK(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
-        const int rc = launch_pathtrace(p->cap_stream, w, nullptr);
+        const int rc = frame(p->cap_stream);
         hipGraph_t g = nullptr;
         const hipError_t ec = hipStreamEndCapture(p->cap_stream, &g);
         if (rc) {
@@ -765,9 +810,11 @@ int32_t solhip_program_create(solhip_ctx ctx, uint32_t kind, const solhip_spec *
     }
     if (kind == SOLHIP_PROG_PATHTRACE) {
         const uint32_t blk = p->spec.wf_block;
-        if (p->spec.max_depth > 4096 || p->spec.enable_sky > 2 || (blk && (blk % 64 || blk > 256))) {
+        if (p->spec.max_depth > 4096 || p->spec.enable_sky > 2 || (blk && (blk % 64 || blk > 256)) ||
+            p->spec.wf_slices > Program::kMaxSlices) {
             delete p;
-            set_error("pathtrace spec: max_depth <= 4096, enable_sky in {0,1,2}, wf_block a multiple of 64 <= 256");
+            set_error("pathtrace spec: max_depth <= 4096, enable_sky in {0,1,2}, wf_block a multiple of 64 <= 256, "
+                      "wf_slices <= 16");
             return SOLHIP_ERROR_UNSUPPORTED;
         }
     }
@@ -873,11 +920,14 @@ int32_t solhip_trace(solhip_ctx ctx, solhip_stream st, solhip_program ph, solhip
     }
     switch (p->kind_id) {
     case SOLHIP_PROG_AO: {
-        if (p->queue.reserve(npix * sizeof(AoEntry)) || p->qcount.reserve(256) || p->occl.reserve(npix * a.n_ao))
+        const uint64_t rb = ao_ray_bytes(npix, a.n_ao);
+        if (p->queue.reserve(npix * sizeof(AoEntry)) || p->qcount.reserve(256) || p->occl.reserve(npix * a.n_ao) ||
+            (rb && p->aorays.reserve(rb)))
             return SOLHIP_ERROR_OUT_OF_MEMORY;
         a.queue = (AoEntry *)p->queue.ptr;
         a.qcount = (uint32_t *)p->qcount.ptr;
         a.occl = (uint8_t *)p->occl.ptr;
+        a.aorays = rb ? (QRay *)p->aorays.ptr : nullptr;
         a.ncu = c->ncu;
         int rc = launch_fill(s, a.qcount, 2, 0u);
         if (rc) return rc;

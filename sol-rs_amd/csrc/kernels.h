@@ -37,6 +37,8 @@ struct Timer {
     virtual ~Timer() = default;
 };
 
+struct QRay;
+
 struct ProgArgs {
     SceneView sv;
     DevTables tb;
@@ -55,6 +57,7 @@ struct ProgArgs {
     AoEntry *queue;
     uint32_t *qcount;       // [0] queue entries, [1] ray cursor of the persistent AO kernel
     uint8_t *occl;          // per AO ray: occluded (persistent AO kernel)
+    QRay *aorays;           // per AO ray: origin, pixel, direction (written by k_ao_primary; nullptr: none)
     uint32_t ncu;
     // pathtrace
     uint32_t enable_sky, spp, max_depth;
@@ -125,7 +128,8 @@ struct WfArgs {
     uint32_t *out_rgba8;
     uint32_t enable_sky, spp, max_depth, rr_start, nee;
     float rmin, rmax;                 // ray range of every segment (pathtrace.rgen:35)
-    uint32_t nslots;                  // 8x8 pixel tiles * 64: slots of the first queue
+    uint32_t nslots;                  // slots of the first queue (8x8 pixel tiles * 64)
+    uint32_t slot_base;               // first slot of this image slice (0: the whole image)
     // per-pixel path state: (T, rng), (L, depth | count_emission << 16), pixel sum
     float4 *st0, *st1, *psum;
     QRay *q[2];                       // extension queues (ping-pong by bounce)
@@ -147,6 +151,8 @@ struct WfArgs {
 int launch_trace_rays(hipStream_t s, const SceneView &sv, const solhip_ray *rays, solhip_hit *hits, uint32_t n,
                       uint32_t flags, Counters *counters, Timer *timer);
 int launch_ao(hipStream_t s, const ProgArgs &a, Timer *timer);
+// bytes of the AO ray buffer the AO program needs for npix pixels (0: none)
+uint64_t ao_ray_bytes(uint64_t npix, uint32_t n_ao);
 int launch_ao_glsl(hipStream_t s, const ProgArgs &a, Timer *timer);
 int launch_pathtrace_glsl(hipStream_t s, const ProgArgs &a, Timer *timer);
 int launch_debug(hipStream_t s, const ProgArgs &a, Timer *timer);
@@ -169,6 +175,10 @@ struct PassHook {
 // one frame of the wavefront path tracer (kernels only: graph-capturable
 // when the radiance cache is off)
 int launch_pathtrace(hipStream_t s, const WfArgs &a, Timer *timer, const PassHook *hook = nullptr);
+// the same in two parts: the passes over one image slice, and the final
+// image write over the whole image (slot_base 0, all slots)
+int launch_pathtrace_passes(hipStream_t s, const WfArgs &a, Timer *timer, const PassHook *hook = nullptr);
+int launch_pathtrace_finalize(hipStream_t s, const WfArgs &a, Timer *timer);
 // threads per CU that the traversal kernels' register budget keeps resident
 uint32_t wf_threads_per_cu();
 // writes the frame constants through a kernel argument (no host buffer to keep alive)

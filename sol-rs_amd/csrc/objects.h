@@ -112,7 +112,7 @@ struct Tlas : Obj {
 struct Program : Obj {
     uint32_t kind_id = 0;
     solhip_spec spec{};
-    DevBuf queue, qcount, occl;
+    DevBuf queue, qcount, occl, aorays;
     // SOLHIP_PROG_PATHTRACE: path state, queues, emitter table, frame constants
     DevBuf st0, st1, psum, q0, q1, hits, sq, scontrib, counts, heads, lights, light_offsets, fc;
     // radiance cache (spec.nrc)
@@ -126,6 +126,10 @@ struct Program : Obj {
     WfArgs gkey{};
     bool gvalid = false;
     uint64_t graph_captures = 0;
+    // image slices (spec.wf_slices): a stream each, fork and join events
+    static constexpr uint32_t kMaxSlices = 16;
+    hipStream_t slice_stream[kMaxSlices] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[kMaxSlices] = {};
     void drop_graph()
     {
         if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -139,7 +143,15 @@ struct Program : Obj {
         drop_graph();
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
         cap_stream = nullptr;
-        for (DevBuf *b : {&queue, &qcount, &occl, &st0, &st1, &psum, &q0, &q1, &hits, &sq, &scontrib, &counts, &heads,
+        for (uint32_t j = 0; j < kMaxSlices; ++j) {
+            if (slice_stream[j]) (void)hipStreamDestroy(slice_stream[j]);
+            if (join_ev[j]) (void)hipEventDestroy(join_ev[j]);
+            slice_stream[j] = nullptr;
+            join_ev[j] = nullptr;
+        }
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
+        fork_ev = nullptr;
+        for (DevBuf *b : {&queue, &qcount, &occl, &aorays, &st0, &st1, &psum, &q0, &q1, &hits, &sq, &scontrib, &counts, &heads,
                           &lights, &light_offsets, &fc, &nq_feat, &nq_rec, &n_counts, &n_tidx, &n_tpath, &n_trec_n,
                           &n_tfeat, &n_trec, &n_bfeat, &n_btarget})
             b->release();

@@ -31,6 +31,11 @@ namespace solhip {
 #ifndef SOLHIP_AO_REFILL
 #define SOLHIP_AO_REFILL 40
 #endif
+// 1: k_ao_primary writes every AO ray (32 B) and the persistent kernel's
+// refill only loads it; 0: the refill samples the direction itself
+#ifndef SOLHIP_AO_RAYBUF
+#define SOLHIP_AO_RAYBUF 1
+#endif
 
 // ---------------------------------------------------------------------------
 // plain ray-traced AO (wavefront)
@@ -95,6 +100,28 @@ __global__ void __launch_bounds__(kProgBlock) k_ao_primary(ProgArgs A)
             const uint32_t slot = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
             A.queue[slot] = e;
+            if (A.aorays) {
+                // the hit's AO rays in order: two numbers each from the
+                // pixel's stream (ao.rchit's cosine lobe around the normal)
+                uint32_t rng = e.rng;
+                const V3 n = v3(e.nx, e.ny, e.nz);
+                QRay *dst = A.aorays + (uint64_t)slot * A.n_ao;
+                for (uint32_t k = 0; k < A.n_ao; ++k) {
+                    const float xi0 = next_rand(rng);
+                    const float xi1 = next_rand(rng);
+                    const V3 dir = sample_cosine(n, xi0, xi1);
+                    QRay r;
+                    r.ox = e.px;
+                    r.oy = e.py;
+                    r.oz = e.pz;
+                    r.path = e.pixel;
+                    r.dx = dir.x;
+                    r.dy = dir.y;
+                    r.dz = dir.z;
+                    r.tmax = A.ao_tmax;
+                    dst[k] = r;
+                }
+            }
         }
     }
     count_rays<COUNT>(A.counters + SOLHIP_KERNEL_AO_PRIMARY, active ? 1u : 0u, tc);
@@ -175,14 +202,20 @@ __global__ void __launch_bounds__(kProgBlock) SOLHIP_AO_RAYS_OCCUPANCY k_ao_rays
             if (!has) {
                 const uint32_t my = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
                 if (my < total) {
-                    const AoEntry e = A.queue[my >> shift];
-                    uint32_t rng = pcg_advance(e.rng, 2u * (my & (A.n_ao - 1u)));
-                    const float xi0 = next_rand(rng);
-                    const float xi1 = next_rand(rng);
-                    const V3 dir = sample_cosine(v3(e.nx, e.ny, e.nz), xi0, xi1);
-                    trav_init(st, A.sv, v3(e.px, e.py, e.pz), dir, A.ao_tmin, A.ao_tmax);
+                    if (A.aorays) {
+                        const QRay r = A.aorays[my];
+                        trav_init(st, A.sv, v3(r.ox, r.oy, r.oz), v3(r.dx, r.dy, r.dz), A.ao_tmin, A.ao_tmax);
+                        pix = r.path;
+                    } else {
+                        const AoEntry e = A.queue[my >> shift];
+                        uint32_t rng = pcg_advance(e.rng, 2u * (my & (A.n_ao - 1u)));
+                        const float xi0 = next_rand(rng);
+                        const float xi1 = next_rand(rng);
+                        const V3 dir = sample_cosine(v3(e.nx, e.ny, e.nz), xi0, xi1);
+                        trav_init(st, A.sv, v3(e.px, e.py, e.pz), dir, A.ao_tmin, A.ao_tmax);
+                        pix = e.pixel;
+                    }
                     rid = my;
-                    pix = e.pixel;
                     has = true;
                 }
             }
@@ -235,6 +268,11 @@ static int launch_status(const char *what)
             hipLaunchKernelGGL(kernel<false>, dim3(grid), dim3(kProgBlock), stack_bytes(a), s, a); \
         if (timer) (timer)->end(s, id);                                                       \
     } while (0)
+
+uint64_t ao_ray_bytes(uint64_t npix, uint32_t n_ao)
+{
+    return SOLHIP_AO_RAYBUF && SOLHIP_AO_REFILL ? npix * n_ao * sizeof(QRay) : 0;
+}
 
 int launch_ao(hipStream_t s, const ProgArgs &a, Timer *timer)
 {

@@ -78,7 +78,7 @@ __global__ void k_pt_generate(WfArgs A, uint32_t pass)
     if (slot >= A.nslots) return;
     uint32_t x, y;
     QRay r;
-    if (!slot_pixel(slot, A.W, A.H, x, y)) {
+    if (!slot_pixel(A.slot_base + slot, A.W, A.H, x, y)) {
         r.ox = r.oy = r.oz = r.dx = r.dy = r.dz = r.tmax = 0.0f;
         r.path = kNoPath;
         A.q[0][slot] = r;
@@ -473,7 +473,7 @@ __global__ void k_pt_finalize(WfArgs A)
 {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t x, y;
-    if (slot >= A.nslots || !slot_pixel(slot, A.W, A.H, x, y)) return;
+    if (slot >= A.nslots || !slot_pixel(A.slot_base + slot, A.W, A.H, x, y)) return;
     const uint32_t p = y * A.W + x;
     const float4 s1 = A.st1[p];
     const float4 ps = A.spp == 1 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : A.psum[p];
@@ -639,7 +639,23 @@ int launch_set_frame(hipStream_t s, FrameConsts *dst, const FrameConsts &value)
     return launch_status("frame constants");
 }
 
-int launch_pathtrace(hipStream_t s, const WfArgs &a0, Timer *timer, const PassHook *hook)
+int launch_pathtrace(hipStream_t s, const WfArgs &a, Timer *timer, const PassHook *hook)
+{
+    const int rc = launch_pathtrace_passes(s, a, timer, hook);
+    return rc ? rc : launch_pathtrace_finalize(s, a, timer);
+}
+
+int launch_pathtrace_finalize(hipStream_t s, const WfArgs &a, Timer *timer)
+{
+    const uint32_t blk = a.block;
+    const uint32_t fin_blocks = (a.nslots + blk - 1) / blk;
+    if (timer) timer->begin(s, SOLHIP_KERNEL_PT_FINALIZE);
+    hipLaunchKernelGGL(k_pt_finalize, dim3(fin_blocks), dim3(blk), 0, s, a);
+    if (timer) timer->end(s, SOLHIP_KERNEL_PT_FINALIZE);
+    return launch_status("pathtrace finalize");
+}
+
+int launch_pathtrace_passes(hipStream_t s, const WfArgs &a0, Timer *timer, const PassHook *hook)
 {
     // the step traversal may hold two more stack entries than trace_ray
     WfArgs a = a0;
@@ -648,7 +664,6 @@ int launch_pathtrace(hipStream_t s, const WfArgs &a0, Timer *timer, const PassHo
     const size_t lds = (size_t)a.sv.stack_depth * sizeof(uint2) * blk;
     const uint32_t nheads = 2 * (a.max_depth + 1);
     const uint32_t gen_blocks = (std::max(a.nslots, nheads) + blk - 1) / blk;
-    const uint32_t fin_blocks = (a.nslots + blk - 1) / blk;
     const bool shadows = a.nee && a.nlights > 0;
     auto timed = [&](int id, auto &&launch) {
         if (timer) timer->begin(s, id);
@@ -688,8 +703,7 @@ int launch_pathtrace(hipStream_t s, const WfArgs &a0, Timer *timer, const PassHo
         int rc = launch_status("pathtrace pass");
         if (rc) return rc;
     }
-    timed(SOLHIP_KERNEL_PT_FINALIZE, [&] { hipLaunchKernelGGL(k_pt_finalize, dim3(fin_blocks), dim3(blk), 0, s, a); });
-    return launch_status("pathtrace finalize");
+    return SOLHIP_OK;
 }
 
 }  // namespace solhip

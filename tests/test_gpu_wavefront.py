@@ -80,6 +80,24 @@ def test_launch_shapes_do_not_change_the_image(ctx, block, per_cu):
     assert (_bits(a0) != _bits(a1)).sum() == 0 and n0 == n1
 
 
+@pytest.mark.parametrize("slices,graph", [(2, 0), (3, 0), (16, 0), (3, 1), (16, 1)])
+def test_image_slices_do_not_change_the_image(ctx, slices, graph):
+    """the frame split into tile slices on their own streams (and captured
+    as one graph with a branch per slice) equals the one-stream frame, on
+    an image whose 24 tiles do not split evenly"""
+    sd = scenes.random_instances()
+    gs = solhip.GpuScene(ctx, sd)
+    W, H = 45, 27
+    print("slices:", slices, graph, flush=True)
+    s0 = _spec(solhip.PROG_PATHTRACE, sky=1, spp=3, depth=6, nee=1, rr=2)
+    s0.wf_slices = 1
+    s1 = _spec(solhip.PROG_PATHTRACE, sky=1, spp=3, depth=6, nee=1, rr=2)
+    s1.wf_slices, s1.wf_graph = slices, graph
+    a0, o0, n0 = _render(ctx, gs, sd, solhip.PROG_PATHTRACE, s0, W, H, 3)
+    a1, o1, n1 = _render(ctx, gs, sd, solhip.PROG_PATHTRACE, s1, W, H, 3)
+    assert (_bits(a0) != _bits(a1)).sum() == 0 and np.array_equal(o0, o1) and n0 == n1 > 0
+
+
 @pytest.mark.parametrize("name,sky,nee,rr,depth", [("cornell", 0, 1, 0, 4), ("cornell", 0, 1, 2, 8),
                                                   ("cornell", 1, 0, 1, 5), ("instances", 2, 1, 3, 8)])
 def test_extended_wavefront_matches_oracle(ctx, oracle, name, sky, nee, rr, depth):
@@ -124,6 +142,10 @@ def test_bad_specs_are_rejected(ctx):
         s = _spec(solhip.PROG_PATHTRACE, **kw)
         with pytest.raises(solhip.SolhipError):
             ctx.program(solhip.PROG_PATHTRACE, s)
+    s = _spec(solhip.PROG_PATHTRACE)
+    s.wf_slices = 17
+    with pytest.raises(solhip.SolhipError):
+        ctx.program(solhip.PROG_PATHTRACE, s)
 
 
 def _block_means(ctx, gs, sd, spec, W, H, frames, blk=8):
