@@ -184,6 +184,16 @@ typedef struct {
      * independent, so the image does not depend on it.  0 = default (1);
      * always 1 with SOLHIP_PROFILE_TIME or nrc. */
     uint32_t wf_slices;
+    /* SOLHIP_PROG_PATHTRACE random streams.  0: one stream per pixel runs
+     * through all samples of the frame, as in pathtrace.rgen:47 (samples are
+     * traced one after another).  1: sample s of pixel p starts its own
+     * stream tea(p + s * W * H, frame) (sample 0 as before), so several
+     * samples of a pixel can be in flight at once; the image is then
+     * independent of wf_samples_per_pass. */
+    uint32_t sample_streams;
+    /* samples of every pixel traced together per pass (sample_streams = 1
+     * only; 0 = enough for about 16 M paths in flight; always 1 with nrc) */
+    uint32_t wf_samples_per_pass;
 } solhip_spec;
 
 /* per-dispatch images: W*H pixels, row-major, no padding */

@@ -1079,6 +1079,9 @@ static void pt_cfg_pixel(const sr_scene *s, const sr_tables *tb, const sr_unifor
     float pixelColor[3] = {0.0f, 0.0f, 0.0f};
     uint32_t rng = sr_tea(x + y * W, un->frame[2]);
     for (int i = 0; i < sampleCount; i++) {
+        /* pathtrace.rgen:47 seeds once per pixel; with sample_streams every
+         * sample gets its own seed of the same hash (sample 0 unchanged) */
+        if (p->sample_streams) rng = sr_tea(x + y * W + (uint32_t)i * W * H, un->frame[2]);
         float jx = sr_next_rand(&rng);
         float jy = sr_next_rand(&rng);
         float o[3], d[3];

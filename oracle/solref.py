@@ -40,7 +40,7 @@ class PTParams(C.Structure):
 class PTConfig(C.Structure):
     """sr_pt_config: the wavefront path tracer's estimator options"""
     _fields_ = [("enable_sky", C.c_uint32), ("samples", C.c_uint32), ("max_bounces", C.c_uint32),
-                ("rr_start", C.c_uint32), ("nee", C.c_uint32)]
+                ("rr_start", C.c_uint32), ("nee", C.c_uint32), ("sample_streams", C.c_uint32)]
 
 
 LIGHT_DTYPE = np.dtype([("p0", "<f4", 3), ("p1", "<f4", 3), ("p2", "<f4", 3), ("n", "<f4", 3), ("e", "<f4", 3),

@@ -242,6 +242,8 @@ int trace_nrc(Context *c, hipStream_t s, Program *p, Tlas *t, WfArgs &w, Timer *
 // SOLHIP_PROG_PATHTRACE: buffers, emitter table, the wavefront passes
 // image slices of a frame when the spec leaves it open
 constexpr uint32_t kDefaultSlices = 1;
+// paths in flight per pass that the default samples-per-pass aims for
+constexpr uint64_t kPathsInFlight = 16ull << 20;
 
 int trace_wavefront(Context *c, hipStream_t s, Program *p, Tlas *t, const ProgArgs &a)
 {
@@ -258,6 +260,8 @@ int trace_wavefront(Context *c, hipStream_t s, Program *p, Tlas *t, const ProgAr
     w.max_depth = p->spec.max_depth;
     w.rr_start = p->spec.rr_start;
     w.nee = p->spec.nee;
+    w.streams = p->spec.sample_streams ? 1u : 0u;
+    w.npix = a.W * a.H;
     w.rmax = 10000.0f;   // pathtrace.rgen:35
     const uint64_t tiles = (uint64_t)((a.W + 7) / 8) * ((a.H + 7) / 8);
     if (tiles * 64 > 0xFFFFFFF0ull) return bad_arg("image too large");
@@ -270,10 +274,21 @@ int trace_wavefront(Context *c, hipStream_t s, Program *p, Tlas *t, const ProgAr
     // default: the waves per CU that the traversal kernels' registers allow
     const uint32_t per_cu = p->spec.wf_blocks_per_cu ? p->spec.wf_blocks_per_cu : std::max(1u, wf_threads_per_cu() / w.block);
     w.persist_blocks = c->ncu * per_cu;
-    w.shade_blocks = (uint32_t)std::min<uint64_t>((w.nslots + w.block - 1) / w.block, (uint64_t)c->ncu * 2048 / w.block);
+    // samples per pass: one stream per pixel traces them one after another;
+    // with a stream per sample, enough of them for about kPathsInFlight paths
+    uint32_t S = 1;
+    if (w.streams && !p->spec.nrc) {
+        S = p->spec.wf_samples_per_pass ? p->spec.wf_samples_per_pass
+                                         : (uint32_t)std::max<uint64_t>(1, kPathsInFlight / w.nslots);
+        S = std::min(S, w.spp);
+    }
+    if ((uint64_t)S * w.nslots > 0xFFFFFFF0ull) return bad_arg("image too large for wf_samples_per_pass");
+    w.S = S;
+    w.shade_blocks = (uint32_t)std::min<uint64_t>(((uint64_t)S * w.nslots + w.block - 1) / w.block,
+                                                  (uint64_t)c->ncu * 2048 / w.block);
     w.counters = c->counters;
     w.count = a.count;
-    const uint64_t npix = (uint64_t)a.W * a.H, ns = w.nslots;
+    const uint64_t npix = (uint64_t)a.W * a.H, ns = (uint64_t)w.S * w.nslots, np = (uint64_t)w.S * npix;
     const uint64_t nheads = 2ull * (w.max_depth + 1);
     // image slices: profiling times every launch on one stream, and the
     // radiance cache works on the whole image between passes
@@ -281,7 +296,7 @@ int trace_wavefront(Context *c, hipStream_t s, Program *p, Tlas *t, const ProgAr
     uint32_t nslices = p->spec.wf_slices ? p->spec.wf_slices : kDefaultSlices;
     if (p->spec.nrc || c->timer_or_null()) nslices = 1;
     nslices = std::max(1u, std::min(nslices, tiles32));
-    if (p->st0.reserve(npix * 16) || p->st1.reserve(npix * 16) || p->psum.reserve(npix * 16) ||
+    if (p->st0.reserve(np * 16) || p->st1.reserve(np * 16) || p->psum.reserve(npix * 16) ||
         p->q0.reserve(ns * sizeof(QRay)) || p->q1.reserve(ns * sizeof(QRay)) || p->hits.reserve(ns * sizeof(solhip_hit)) ||
         p->sq.reserve(ns * sizeof(QRay)) || p->scontrib.reserve(ns * 16) || p->counts.reserve(256ull * nslices) ||
         p->heads.reserve(nheads * 4 * nslices) || p->fc.reserve(sizeof(FrameConsts)))
@@ -332,14 +347,16 @@ int trace_wavefront(Context *c, hipStream_t s, Program *p, Tlas *t, const ProgAr
             WfArgs ws = w;
             ws.slot_base = t0 * 64u;
             ws.nslots = (t1 - t0) * 64u;
-            ws.q[0] = w.q[0] + ws.slot_base;
-            ws.q[1] = w.q[1] + ws.slot_base;
-            ws.hits = w.hits + ws.slot_base;
-            ws.sq = w.sq + ws.slot_base;
-            ws.scontrib = w.scontrib + ws.slot_base;
+            const uint64_t qb = (uint64_t)w.S * ws.slot_base;   // the slice's part of the queues
+            ws.q[0] = w.q[0] + qb;
+            ws.q[1] = w.q[1] + qb;
+            ws.hits = w.hits + qb;
+            ws.sq = w.sq + qb;
+            ws.scontrib = w.scontrib + qb;
             ws.counts = w.counts + 64u * j;
             ws.heads = w.heads + nheads * j;
-            ws.shade_blocks = (uint32_t)std::min<uint64_t>((ws.nslots + w.block - 1) / w.block, w.shade_blocks);
+            ws.shade_blocks = (uint32_t)std::min<uint64_t>(((uint64_t)w.S * ws.nslots + w.block - 1) / w.block,
+                                                           w.shade_blocks);
             hipStream_t ss = p->slice_stream[j];
             SOLHIP_HIP_CHECK(hipStreamWaitEvent(ss, p->fork_ev, 0));
             const int rc = launch_pathtrace_passes(ss, ws, nullptr);
@@ -811,10 +828,11 @@ int32_t solhip_program_create(solhip_ctx ctx, uint32_t kind, const solhip_spec *
     if (kind == SOLHIP_PROG_PATHTRACE) {
         const uint32_t blk = p->spec.wf_block;
         if (p->spec.max_depth > 4096 || p->spec.enable_sky > 2 || (blk && (blk % 64 || blk > 256)) ||
-            p->spec.wf_slices > Program::kMaxSlices) {
+            p->spec.wf_slices > Program::kMaxSlices || p->spec.sample_streams > 1 ||
+            (p->spec.wf_samples_per_pass > 1 && !p->spec.sample_streams)) {
             delete p;
             set_error("pathtrace spec: max_depth <= 4096, enable_sky in {0,1,2}, wf_block a multiple of 64 <= 256, "
-                      "wf_slices <= 16");
+                      "wf_slices <= 16, sample_streams in {0,1}, wf_samples_per_pass > 1 only with sample_streams");
             return SOLHIP_ERROR_UNSUPPORTED;
         }
     }
@@ -920,10 +938,11 @@ int32_t solhip_trace(solhip_ctx ctx, solhip_stream st, solhip_program ph, solhip
     }
     switch (p->kind_id) {
     case SOLHIP_PROG_AO: {
-        const uint64_t rb = ao_ray_bytes(npix, a.n_ao);
+        const uint64_t rb = ao_ray_bytes(npix, a.n_ao), hb = ao_head_bytes();
         if (p->queue.reserve(npix * sizeof(AoEntry)) || p->qcount.reserve(256) || p->occl.reserve(npix * a.n_ao) ||
-            (rb && p->aorays.reserve(rb)))
+            (rb && p->aorays.reserve(rb)) || (hb && p->aoheads.reserve(hb)))
             return SOLHIP_ERROR_OUT_OF_MEMORY;
+        a.aoheads = hb ? (uint32_t *)p->aoheads.ptr : nullptr;
         a.queue = (AoEntry *)p->queue.ptr;
         a.qcount = (uint32_t *)p->qcount.ptr;
         a.occl = (uint8_t *)p->occl.ptr;

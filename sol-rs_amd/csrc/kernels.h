@@ -58,6 +58,7 @@ struct ProgArgs {
     uint32_t *qcount;       // [0] queue entries, [1] ray cursor of the persistent AO kernel
     uint8_t *occl;          // per AO ray: occluded (persistent AO kernel)
     QRay *aorays;           // per AO ray: origin, pixel, direction (written by k_ao_primary; nullptr: none)
+    uint32_t *aoheads;      // fetch heads of the persistent AO kernel's ray partitions, 128 B apart
     uint32_t ncu;
     // pathtrace
     uint32_t enable_sky, spp, max_depth;
@@ -127,10 +128,14 @@ struct WfArgs {
     float4 *accum;
     uint32_t *out_rgba8;
     uint32_t enable_sky, spp, max_depth, rr_start, nee;
+    uint32_t streams;                 // 1: sample s of pixel p starts tea(p + s * npix, frame)
+    uint32_t S;                       // samples per pass (> 1 only with streams)
+    uint32_t npix;                    // W * H; path index = sample-in-pass * npix + pixel
     float rmin, rmax;                 // ray range of every segment (pathtrace.rgen:35)
     uint32_t nslots;                  // slots of the first queue (8x8 pixel tiles * 64)
     uint32_t slot_base;               // first slot of this image slice (0: the whole image)
-    // per-pixel path state: (T, rng), (L, depth | count_emission << 16), pixel sum
+    // path state, S * npix each: (T, rng), (L, depth | count_emission << 16);
+    // per-pixel sum of the finished samples
     float4 *st0, *st1, *psum;
     QRay *q[2];                       // extension queues (ping-pong by bounce)
     solhip_hit *hits;                 // closest hits, by slot of the current queue
@@ -153,6 +158,8 @@ int launch_trace_rays(hipStream_t s, const SceneView &sv, const solhip_ray *rays
 int launch_ao(hipStream_t s, const ProgArgs &a, Timer *timer);
 // bytes of the AO ray buffer the AO program needs for npix pixels (0: none)
 uint64_t ao_ray_bytes(uint64_t npix, uint32_t n_ao);
+// bytes of the persistent AO kernel's fetch heads
+uint64_t ao_head_bytes();
 int launch_ao_glsl(hipStream_t s, const ProgArgs &a, Timer *timer);
 int launch_pathtrace_glsl(hipStream_t s, const ProgArgs &a, Timer *timer);
 int launch_debug(hipStream_t s, const ProgArgs &a, Timer *timer);

@@ -36,6 +36,14 @@ namespace solhip {
 #ifndef SOLHIP_AO_RAYBUF
 #define SOLHIP_AO_RAYBUF 1
 #endif
+// the AO rays are split into this many contiguous partitions, each with its
+// own fetch head (128 B apart); a wave starts on partition wave % PARTS and
+// moves to the next one when its partition is used up, so refills spread
+// their atomics over PARTS addresses instead of one
+#ifndef SOLHIP_AO_PARTS
+#define SOLHIP_AO_PARTS 64
+#endif
+constexpr uint32_t kHeadStride = 32;   // words between heads
 
 // ---------------------------------------------------------------------------
 // plain ray-traced AO (wavefront)
@@ -44,6 +52,8 @@ template <bool COUNT>
 __global__ void __launch_bounds__(kProgBlock) k_ao_primary(ProgArgs A)
 {
     extern __shared__ uint2 stack[];
+    if (A.aoheads && blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < SOLHIP_AO_PARTS; i += blockDim.x) A.aoheads[i * kHeadStride] = 0;
     uint32_t x, y;
     const bool active = tile_pixel(A.W, A.H, x, y);
     TravCounts tc{0, 0, 0, false};
@@ -191,17 +201,33 @@ __global__ void __launch_bounds__(kProgBlock) SOLHIP_AO_RAYS_OCCUPANCY k_ao_rays
     TravState st;
     bool has = false, exhausted = false;
     uint32_t rid = 0, pix = 0;
+    // ray partitions [total * k / P, total * (k + 1) / P)
+    const uint32_t P = A.aoheads ? (uint32_t)SOLHIP_AO_PARTS : 1u;
+    uint32_t part = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) % P, moved = 0;
+    uint32_t *head = A.aoheads ? A.aoheads + part * kHeadStride : A.qcount + 1;
+    uint32_t lo = (uint32_t)((uint64_t)total * part / P), hi = (uint32_t)((uint64_t)total * (part + 1) / P);
     while (true) {
         const uint64_t idle = __ballot(!has);
         const uint32_t nidle = (uint32_t)__popcll(idle);
         if (!exhausted && nidle >= (uint32_t)SOLHIP_AO_REFILL) {
             uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(A.qcount + 1, nidle);
-            base = __shfl(base, 0);
-            exhausted = base + nidle >= total;
+            if (lane == 0) base = atomicAdd(head, nidle);
+            base = lo + __shfl(base, 0);
+            const uint32_t end = hi;
+            if (base + nidle >= end) {
+                // this partition is used up: the next one, until all were seen
+                if (++moved >= P) {
+                    exhausted = true;
+                } else {
+                    part = part + 1 == P ? 0 : part + 1;
+                    head = A.aoheads + part * kHeadStride;
+                    lo = (uint32_t)((uint64_t)total * part / P);
+                    hi = (uint32_t)((uint64_t)total * (part + 1) / P);
+                }
+            }
             if (!has) {
                 const uint32_t my = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-                if (my < total) {
+                if (my < end) {
                     if (A.aorays) {
                         const QRay r = A.aorays[my];
                         trav_init(st, A.sv, v3(r.ox, r.oy, r.oz), v3(r.dx, r.dy, r.dz), A.ao_tmin, A.ao_tmax);
@@ -220,7 +246,7 @@ __global__ void __launch_bounds__(kProgBlock) SOLHIP_AO_RAYS_OCCUPANCY k_ao_rays
                 }
             }
         }
-        if (__ballot(has) == 0) break;
+        if (__ballot(has) == 0 && exhausted) break;
         if (has && trav_step<ANY, COUNT>(st, A.sv, 0xFFu, stack + threadIdx.x, kProgBlock, tc)) {
             const Hit h = trav_result(st);
             A.occl[rid] = h.inst != SOLHIP_MISS ? 1u : 0u;
@@ -269,6 +295,8 @@ static int launch_status(const char *what)
         if (timer) (timer)->end(s, id);                                                       \
     } while (0)
 
+uint64_t ao_head_bytes() { return SOLHIP_AO_PARTS > 1 && SOLHIP_AO_REFILL ? SOLHIP_AO_PARTS * kHeadStride * 4ull : 0; }
+
 uint64_t ao_ray_bytes(uint64_t npix, uint32_t n_ao)
 {
     return SOLHIP_AO_RAYBUF && SOLHIP_AO_REFILL ? npix * n_ao * sizeof(QRay) : 0;
@@ -311,6 +339,8 @@ template <bool COUNT>
 __global__ void __launch_bounds__(kProgBlock) k_ao_glsl(ProgArgs A)
 {
     extern __shared__ uint2 stack[];
+    if (A.aoheads && blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < SOLHIP_AO_PARTS; i += blockDim.x) A.aoheads[i * kHeadStride] = 0;
     uint32_t x, y;
     const bool active = tile_pixel(A.W, A.H, x, y);
     TravCounts tc{0, 0, 0, false};
@@ -387,6 +417,8 @@ template <bool COUNT>
 __global__ void __launch_bounds__(kProgBlock) k_pathtrace_glsl(ProgArgs A)
 {
     extern __shared__ uint2 stack[];
+    if (A.aoheads && blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < SOLHIP_AO_PARTS; i += blockDim.x) A.aoheads[i * kHeadStride] = 0;
     uint32_t x, y;
     const bool active = tile_pixel(A.W, A.H, x, y);
     TravCounts tc{0, 0, 0, false};
@@ -498,6 +530,8 @@ template <bool COUNT>
 __global__ void __launch_bounds__(kProgBlock) k_debug(ProgArgs A)
 {
     extern __shared__ uint2 stack[];
+    if (A.aoheads && blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < SOLHIP_AO_PARTS; i += blockDim.x) A.aoheads[i * kHeadStride] = 0;
     uint32_t x, y;
     const bool active = tile_pixel(A.W, A.H, x, y);
     TravCounts tc{0, 0, 0, false};

@@ -57,8 +57,8 @@ struct JobDev {
     float total;
     uint32_t W, H, frame, spp, max_depth, rr_start, enable_sky, nee;
     float alpha, rmax;
-    uint32_t base;   // first slot
-    uint32_t pad;
+    uint32_t base;      // first slot
+    uint32_t streams;   // spec.sample_streams: sample s starts tea(p + s * W * H, frame)
 };
 
 struct SrvArgs {
@@ -293,7 +293,7 @@ __global__ void __launch_bounds__(kSrvBlock) k_srv_retire(SrvArgs A, uint32_t cu
             const float pc0 = ps.x + s1.x, pc1 = ps.y + s1.y, pc2 = ps.z + s1.z;
             if (sample < J.spp) {
                 A.psum[slot] = make_float4(pc0, pc1, pc2, 0.0f);
-                uint32_t rng = as_u(A.st0[slot].w);
+                uint32_t rng = J.streams ? tea(p + sample * (J.W * J.H), J.frame) : as_u(A.st0[slot].w);
                 float4 s0n, s1n;
                 r = start_sample(J, slot, x, y, rng, s0n, s1n);
                 A.st0[slot] = s0n;
@@ -482,6 +482,7 @@ struct Server : Obj {
             v.rr_start = jb.desc.spec.rr_start;
             v.enable_sky = jb.desc.spec.enable_sky;
             v.nee = jb.desc.spec.nee;
+            v.streams = jb.desc.spec.sample_streams;
             const uint32_t denom = jb.desc.uniforms.frame[2] - (uint32_t)jb.desc.accum_start_frame + 1u;
             v.alpha = 1.0f / (float)denom;   // pathtrace.rgen:90
             v.rmax = 10000.0f;
@@ -678,7 +679,7 @@ int32_t solhip_server_submit(solhip_ctx ctx, solhip_server h, const solhip_job_d
     const uint64_t npix = (uint64_t)job->W * job->H;
     if (npix > sv->max_slots) return bad_arg("image larger than the server's slot budget");
     if (job->spec.spp_per_frame == 0 || job->spec.spp_per_frame > 0xFFFFFFu || job->spec.max_depth > 4096 ||
-        job->spec.enable_sky > 2)
+        job->spec.enable_sky > 2 || job->spec.sample_streams > 1)
         return bad_arg("job spec");
     Buffer *acc = get<Buffer>(job->accum, Kind::Buffer);
     if (!acc || acc->bytes < npix * 16) return bad_arg("job accum buffer");

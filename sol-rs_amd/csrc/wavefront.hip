@@ -66,34 +66,50 @@ __device__ __forceinline__ float ray_tmin(const WfArgs &A)
     return fmaxf(1.0f, sqrtf(o[0] * o[0] + o[1] * o[1] + o[2] * o[2])) * 1e-3f;
 }
 
+// samples traced in pass `pass` (the last pass may hold fewer than S)
+__device__ __forceinline__ uint32_t pass_samples(const WfArgs &A, uint32_t pass)
+{
+    return min(A.S, A.spp - pass * A.S);
+}
+
+// first queue of pass `pass`: entry e = sample-in-pass * nslots + slot, path
+// index = sample-in-pass * npix + pixel
 __global__ void k_pt_generate(WfArgs A, uint32_t pass)
 {
-    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot == 0) {
-        A.counts[0] = A.nslots;
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = pass_samples(A, pass) * A.nslots;
+    if (e == 0) {
+        A.counts[0] = n;
         A.counts[1] = 0;
         A.counts[2] = 0;
     }
-    if (slot < 2 * (A.max_depth + 1)) A.heads[slot] = 0;
-    if (slot >= A.nslots) return;
+    if (e < 2 * (A.max_depth + 1)) A.heads[e] = 0;
+    if (e >= n) return;
+    const uint32_t sl = e / A.nslots, slot = e - sl * A.nslots;
     uint32_t x, y;
     QRay r;
     if (!slot_pixel(A.slot_base + slot, A.W, A.H, x, y)) {
         r.ox = r.oy = r.oz = r.dx = r.dy = r.dz = r.tmax = 0.0f;
         r.path = kNoPath;
-        A.q[0][slot] = r;
+        A.q[0][e] = r;
         return;
     }
-    const uint32_t p = y * A.W + x;
+    const uint32_t pix = y * A.W + x, p = sl * A.npix + pix;
     uint32_t rng;
-    if (pass == 0) {
+    if (A.streams) {
+        // sample s of the pixel starts its own stream (s = 0: pathtrace.rgen:47)
+        rng = tea(pix + (pass * A.S + sl) * A.npix, A.fc->frame);
+    } else if (pass == 0) {
         rng = tea(x + y * A.W, A.fc->frame);   // pathtrace.rgen:47
     } else {
-        // the RNG runs on from the previous sample; its radiance joins the sum
-        const float4 s0 = A.st0[p], s1 = A.st1[p];
-        rng = as_u(s0.w);
-        const float4 ps = pass == 1 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : A.psum[p];
-        A.psum[p] = make_float4(ps.x + s1.x, ps.y + s1.y, ps.z + s1.z, 0.0f);
+        // one sample per pass: the RNG runs on from the previous sample
+        rng = as_u(A.st0[p].w);
+    }
+    if (A.S == 1 && pass > 0) {
+        // the previous sample's radiance joins the sum
+        const float4 s1 = A.st1[p];
+        const float4 ps = pass == 1 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : A.psum[pix];
+        A.psum[pix] = make_float4(ps.x + s1.x, ps.y + s1.y, ps.z + s1.z, 0.0f);
     }
     const float jx = next_rand(rng);
     const float jy = next_rand(rng);   // :52
@@ -109,7 +125,25 @@ __global__ void k_pt_generate(WfArgs A, uint32_t pass)
     r.dy = d.y;
     r.dz = d.z;
     r.tmax = A.rmax;
-    A.q[0][slot] = r;
+    A.q[0][e] = r;
+}
+
+// the radiance of pass `pass`'s samples joins each pixel's sum, in sample
+// order (the order of the one-sample-per-pass sums); S > 1 only, before the
+// next pass's generate overwrites the path state
+__global__ void k_pt_fold(WfArgs A, uint32_t pass)
+{
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t x, y;
+    if (slot >= A.nslots || !slot_pixel(A.slot_base + slot, A.W, A.H, x, y)) return;
+    const uint32_t pix = y * A.W + x;
+    float4 ps = pass == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : A.psum[pix];
+    const uint32_t sk = pass_samples(A, pass);
+    for (uint32_t sl = 0; sl < sk; ++sl) {
+        const float4 s1 = A.st1[sl * A.npix + pix];
+        ps = make_float4(ps.x + s1.x, ps.y + s1.y, ps.z + s1.z, 0.0f);
+    }
+    A.psum[pix] = ps;
 }
 
 // register budget of the traversal kernels: waves per SIMD they must fit
@@ -475,9 +509,14 @@ __global__ void k_pt_finalize(WfArgs A)
     uint32_t x, y;
     if (slot >= A.nslots || !slot_pixel(A.slot_base + slot, A.W, A.H, x, y)) return;
     const uint32_t p = y * A.W + x;
-    const float4 s1 = A.st1[p];
-    const float4 ps = A.spp == 1 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : A.psum[p];
-    float pc0 = ps.x + s1.x, pc1 = ps.y + s1.y, pc2 = ps.z + s1.z;
+    // the last pass's samples join the sum of the earlier ones, in order
+    const uint32_t passes = (A.spp + A.S - 1) / A.S, sk = A.spp - (passes - 1) * A.S;
+    float4 ps = passes == 1 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : A.psum[p];
+    for (uint32_t sl = 0; sl < sk; ++sl) {
+        const float4 s1 = A.st1[sl * A.npix + p];
+        ps = make_float4(ps.x + s1.x, ps.y + s1.y, ps.z + s1.z, 0.0f);
+    }
+    float pc0 = ps.x, pc1 = ps.y, pc2 = ps.z;
     const float inv = 1.0f / (float)A.spp;   // pathtrace.rgen:88
     pc0 = inv * pc0;
     pc1 = inv * pc1;
@@ -663,7 +702,9 @@ int launch_pathtrace_passes(hipStream_t s, const WfArgs &a0, Timer *timer, const
     const uint32_t blk = a.block;
     const size_t lds = (size_t)a.sv.stack_depth * sizeof(uint2) * blk;
     const uint32_t nheads = 2 * (a.max_depth + 1);
-    const uint32_t gen_blocks = (std::max(a.nslots, nheads) + blk - 1) / blk;
+    const uint32_t gen_blocks = (std::max(a.S * a.nslots, nheads) + blk - 1) / blk;
+    const uint32_t fold_blocks = (a.nslots + blk - 1) / blk;
+    const uint32_t passes = (a.spp + a.S - 1) / a.S;
     const bool shadows = a.nee && a.nlights > 0;
     auto timed = [&](int id, auto &&launch) {
         if (timer) timer->begin(s, id);
@@ -671,7 +712,10 @@ int launch_pathtrace_passes(hipStream_t s, const WfArgs &a0, Timer *timer, const
         if (timer) timer->end(s, id);
     };
     const bool nrc = a.nrc.enabled != 0;
-    for (uint32_t pass = 0; pass < a.spp; ++pass) {
+    for (uint32_t pass = 0; pass < passes; ++pass) {
+        if (a.S > 1 && pass > 0)
+            timed(SOLHIP_KERNEL_PT_GENERATE,
+                  [&] { hipLaunchKernelGGL(k_pt_fold, dim3(fold_blocks), dim3(blk), 0, s, a, pass - 1); });
         timed(SOLHIP_KERNEL_PT_GENERATE,
               [&] { hipLaunchKernelGGL(k_pt_generate, dim3(gen_blocks), dim3(blk), 0, s, a, pass); });
         if (nrc) {

@@ -45,7 +45,8 @@ class Spec(C.Structure):
                 ("n_ao", C.c_uint32), ("ao_tmin", C.c_float), ("ao_tmax", C.c_float), ("ao_any_hit", C.c_uint32),
                 ("rr_start", C.c_uint32), ("nee", C.c_uint32), ("wf_blocks_per_cu", C.c_uint32),
                 ("wf_block", C.c_uint32), ("wf_graph", C.c_uint32), ("nrc", C.c_uint32),
-                ("nrc_train_steps", C.c_uint32), ("nrc_learning_rate", C.c_float), ("wf_slices", C.c_uint32)]
+                ("nrc_train_steps", C.c_uint32), ("nrc_learning_rate", C.c_float), ("wf_slices", C.c_uint32),
+                ("sample_streams", C.c_uint32), ("wf_samples_per_pass", C.c_uint32)]
 
 
 class Images(C.Structure):

@@ -41,7 +41,7 @@ def test_record_sizes_match_the_reference():
     assert records.UNIFORMS.itemsize == 400         # examples/*.rs:9-17
     assert C.sizeof(api.Geometry) == 48
     assert C.sizeof(api.SceneTables) == 48
-    assert C.sizeof(api.Spec) == 64
+    assert C.sizeof(api.Spec) == 72
 
 
 def test_abi_version_and_spec_defaults():

@@ -14,9 +14,10 @@ from solhip import scenes
 pytestmark = pytest.mark.gpu
 
 
-def _spec(sky=0, spp=3, depth=4, rr=0, nee=0):
+def _spec(sky=0, spp=3, depth=4, rr=0, nee=0, streams=0):
     s = solhip.spec_default(solhip.PROG_PATHTRACE)
     s.enable_sky, s.spp_per_frame, s.max_depth, s.rr_start, s.nee = sky, spp, depth, rr, nee
+    s.sample_streams = streams
     return s
 
 
@@ -36,6 +37,7 @@ JOBS = [  # (scene, W, H, spec kwargs, frame)
     ("cornell", 17, 13, dict(spp=5, depth=3, nee=1), 2),
     ("instances", 33, 20, dict(sky=2, spp=4, depth=8, nee=1, rr=3), 3),
     ("cornell", 64, 48, dict(spp=1, depth=5), 4),
+    ("instances", 30, 22, dict(sky=2, spp=4, depth=6, nee=1, rr=2, streams=1), 5),
 ]
 
 
@@ -109,6 +111,8 @@ def test_bad_submissions(ctx):
         srv.submit(sid, 4096, 4096, _spec(), sd.uniforms(8, 8, 0), acc)     # beyond the slot budget
     with pytest.raises(solhip.SolhipError):
         srv.submit(sid, 8, 8, _spec(spp=0), sd.uniforms(8, 8, 0), acc)
+    with pytest.raises(solhip.SolhipError):
+        srv.submit(sid, 8, 8, _spec(streams=2), sd.uniforms(8, 8, 0), acc)
     with pytest.raises(solhip.SolhipError):
         solhip.Server(ctx, 1 << 20, 0)
     assert srv.run() == 0

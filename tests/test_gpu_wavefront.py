@@ -117,6 +117,32 @@ def test_extended_wavefront_matches_oracle(ctx, oracle, name, sky, nee, rr, dept
     assert n == nref
 
 
+@pytest.mark.parametrize("name,per_pass,slices,graph", [("cornell", 1, 1, 0), ("cornell", 2, 1, 0), ("cornell", 5, 3, 1),
+                                                        ("instances", 0, 1, 0), ("instances", 3, 2, 1)])
+def test_sample_streams_match_oracle(ctx, oracle, name, per_pass, slices, graph):
+    """sample_streams = 1: every pass traces per_pass samples of every pixel
+    at once (5 samples: a short last pass for 2 and 3); the image equals
+    the oracle's per-pixel loop with the same seeding, bit for bit"""
+    sd = SCENES[name]()
+    gs = solhip.GpuScene(ctx, sd)
+    W, H, spp, F = 36, 28, 5, 2
+    sky, nee, rr, depth = (0, 1, 2, 6) if name == "cornell" else (2, 1, 3, 8)
+    print("sample streams vs oracle:", name, per_pass, slices, graph, flush=True)
+    s = _spec(solhip.PROG_PATHTRACE, sky=sky, spp=spp, depth=depth, rr=rr, nee=nee)
+    s.sample_streams, s.wf_samples_per_pass, s.wf_slices, s.wf_graph = 1, per_pass, slices, graph
+    a, o, n = _render(ctx, gs, sd, solhip.PROG_PATHTRACE, s, W, H, F)
+    sc = oracle.Scene(sd)
+    ref = np.zeros((H, W, 4), np.float32)
+    ref8 = np.zeros((H, W, 4), np.uint8)
+    nref = 0
+    for f in range(F):
+        nref += sc.render_pathtrace(sd.uniforms(W, H, f), oracle.PTConfig(sky, spp, depth, rr, nee, 1), 0, ref, ref8,
+                                    W, H)
+    assert (_bits(a) != _bits(ref)).sum() == 0
+    assert np.abs(o.astype(int) - ref8.astype(int)).max() <= 1
+    assert n == nref
+
+
 def test_depth_zero_and_single_sample(ctx, oracle):
     sd = scenes.cornell_box()
     gs = solhip.GpuScene(ctx, sd)
@@ -142,10 +168,11 @@ def test_bad_specs_are_rejected(ctx):
         s = _spec(solhip.PROG_PATHTRACE, **kw)
         with pytest.raises(solhip.SolhipError):
             ctx.program(solhip.PROG_PATHTRACE, s)
-    s = _spec(solhip.PROG_PATHTRACE)
-    s.wf_slices = 17
-    with pytest.raises(solhip.SolhipError):
-        ctx.program(solhip.PROG_PATHTRACE, s)
+    for field, value in (("wf_slices", 17), ("sample_streams", 2), ("wf_samples_per_pass", 4)):
+        s = _spec(solhip.PROG_PATHTRACE)
+        setattr(s, field, value)
+        with pytest.raises(solhip.SolhipError):
+            ctx.program(solhip.PROG_PATHTRACE, s)
 
 
 def _block_means(ctx, gs, sd, spec, W, H, frames, blk=8):

@@ -360,6 +360,29 @@ def test_nee_and_roulette_keep_the_expectation(oracle, nee, rr):
     assert (se < 0.01 * m1).all()   # the band is tight
 
 
+def test_sample_streams_keep_sample_zero_and_the_expectation(oracle):
+    """sample_streams = 1 seeds sample i with tea(x + y*W + i*W*H, frame):
+    sample 0 is the sequential stream's, so one-sample images are identical;
+    more samples differ from the sequential stream but keep its expectation
+    (image means within 5 standard errors)"""
+    sd = _lit_box()
+    W, H = 16, 16
+    sc = oracle.Scene(sd)
+    a = np.zeros((H, W, 4), np.float32)
+    b = np.zeros((H, W, 4), np.float32)
+    sc.render_pathtrace(sd.uniforms(W, H, 3), oracle.PTConfig(0, 1, 3, 2, 1, 0), 0, a, None, W, H)
+    sc.render_pathtrace(sd.uniforms(W, H, 3), oracle.PTConfig(0, 1, 3, 2, 1, 1), 0, b, None, W, H)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    sc.render_pathtrace(sd.uniforms(W, H, 3), oracle.PTConfig(0, 4, 3, 2, 1, 0), 0, a, None, W, H)
+    sc.render_pathtrace(sd.uniforms(W, H, 3), oracle.PTConfig(0, 4, 3, 2, 1, 1), 0, b, None, W, H)
+    assert not np.array_equal(a, b)
+    F = 24
+    seq = _frame_means(oracle, sd, oracle.PTConfig(0, 32, 3, 2, 1, 0), W, H, F)
+    per = _frame_means(oracle, sd, oracle.PTConfig(0, 32, 3, 2, 1, 1), W, H, F)
+    se = np.sqrt(seq.var(0, ddof=1) / F + per.var(0, ddof=1) / F)
+    assert (np.abs(seq.mean(0) - per.mean(0)) < 5 * se).all(), (seq.mean(0), per.mean(0), se)
+
+
 def test_nee_reduces_noise_for_a_small_light(oracle):
     """Cornell box (light 6 % of the ceiling), direct light only: sampling
     the emitter beats waiting for a cosine-sampled ray to hit it"""
