@@ -67,6 +67,10 @@ def parse_args():
                     choices=["ao", "pathtrace-c0", "pathtrace-c2", "server-c3", "cache-c4"])
     ap.add_argument("--jobs", type=int, default=40, help="server-c3: requests")
     ap.add_argument("--rate", type=float, default=200.0, help="server-c3: requests per second")
+    ap.add_argument("--streams", type=int, default=1, choices=(0, 1),
+                    help="pathtrace-c0/c2: 1 = a random stream per sample, 0 = one per pixel (pathtrace.rgen)")
+    ap.add_argument("--per-pass", type=int, default=0, help="pathtrace-c0/c2: samples per pass (0 = library default)")
+    ap.add_argument("--slices", type=int, default=0, help="pathtrace-c0/c2: image slices (0 = library default)")
     ap.add_argument("--any-hit", action="store_true",
                     help="AO rays stop at the first hit (same image; hit records are not the closest)")
     return ap.parse_args()
@@ -261,6 +265,12 @@ def bench_pathtrace(a):
     else:
         spec.enable_sky, spec.spp_per_frame, spec.max_depth, spec.nee, spec.rr_start = 2, 64, 8, 1, 3
     spec.wf_graph = 1
+    # one random stream per sample (sample_streams = 1) lets a pass trace many
+    # samples of every pixel at once; 0 keeps pathtrace.rgen's one stream per
+    # pixel, whose samples must follow one another
+    spec.sample_streams = a.streams
+    spec.wf_samples_per_pass = a.per_pass
+    spec.wf_slices = a.slices
     prog = ctx.program(solhip.PROG_PATHTRACE, spec)
     acc = ctx.upload(np.zeros((H, W, 4), np.float32))
     img = solhip.Images(acc.h, 0, 0, 0, 0, 0)
@@ -298,8 +308,9 @@ def bench_pathtrace(a):
                                 "depth 8, Russian roulette from bounce 3, next-event estimation (no emitters), "
                                 "gradient sky, all BLAS + TLAS rebuilt per frame" % sd.tri_count),
                    "resolution": [W, H], "spp": spec.spp_per_frame, "max_depth": spec.max_depth,
-                   "parallelism": "single GPU"},
-        "extra": {"rays_per_frame": int(st.rays / a.steps), "hipgraph": True},
+                   "sample_streams": a.streams, "parallelism": "single GPU"},
+        "extra": {"rays_per_frame": int(st.rays / a.steps), "hipgraph": True,
+                  "samples_per_pass": a.per_pass or "default", "slices": a.slices or "default"},
     }
     ctx.close()
     print(json.dumps(result))

@@ -181,8 +181,8 @@ typedef struct {
     /* SOLHIP_PROG_PATHTRACE: the image's 8x8 tiles split into this many
      * contiguous slices whose passes run on their own streams at the same
      * time, so one slice's slowest rays overlap the others' work; pixels are
-     * independent, so the image does not depend on it.  0 = default (1);
-     * always 1 with SOLHIP_PROFILE_TIME or nrc. */
+     * independent, so the image does not depend on it.  0 = default (4
+     * with sample_streams, else 1); always 1 with SOLHIP_PROFILE_TIME or nrc. */
     uint32_t wf_slices;
     /* SOLHIP_PROG_PATHTRACE random streams.  0: one stream per pixel runs
      * through all samples of the frame, as in pathtrace.rgen:47 (samples are
@@ -192,7 +192,7 @@ typedef struct {
      * independent of wf_samples_per_pass. */
     uint32_t sample_streams;
     /* samples of every pixel traced together per pass (sample_streams = 1
-     * only; 0 = enough for about 16 M paths in flight; always 1 with nrc) */
+     * only; 0 = enough for about 64 M paths in flight; always 1 with nrc) */
     uint32_t wf_samples_per_pass;
 } solhip_spec;
 

@@ -240,10 +240,13 @@ int trace_nrc(Context *c, hipStream_t s, Program *p, Tlas *t, WfArgs &w, Timer *
 }
 
 // SOLHIP_PROG_PATHTRACE: buffers, emitter table, the wavefront passes
-// image slices of a frame when the spec leaves it open
-constexpr uint32_t kDefaultSlices = 1;
+// image slices of a frame when the spec leaves it open: with a stream per
+// sample, passes are large and slices overlap one another's slowest rays;
+// with one stream per pixel, passes are one sample each and slices only add
+// launches
+constexpr uint32_t kDefaultSlicesStreams = 4;
 // paths in flight per pass that the default samples-per-pass aims for
-constexpr uint64_t kPathsInFlight = 16ull << 20;
+constexpr uint64_t kPathsInFlight = 64ull << 20;
 
 int trace_wavefront(Context *c, hipStream_t s, Program *p, Tlas *t, const ProgArgs &a)
 {
@@ -293,13 +296,13 @@ int trace_wavefront(Context *c, hipStream_t s, Program *p, Tlas *t, const ProgAr
     // image slices: profiling times every launch on one stream, and the
     // radiance cache works on the whole image between passes
     const uint32_t tiles32 = (uint32_t)tiles;
-    uint32_t nslices = p->spec.wf_slices ? p->spec.wf_slices : kDefaultSlices;
+    uint32_t nslices = p->spec.wf_slices ? p->spec.wf_slices : (p->spec.sample_streams ? kDefaultSlicesStreams : 1u);
     if (p->spec.nrc || c->timer_or_null()) nslices = 1;
     nslices = std::max(1u, std::min(nslices, tiles32));
     if (p->st0.reserve(np * 16) || p->st1.reserve(np * 16) || p->psum.reserve(npix * 16) ||
         p->q0.reserve(ns * sizeof(QRay)) || p->q1.reserve(ns * sizeof(QRay)) || p->hits.reserve(ns * sizeof(solhip_hit)) ||
         p->sq.reserve(ns * sizeof(QRay)) || p->scontrib.reserve(ns * 16) || p->counts.reserve(256ull * nslices) ||
-        p->heads.reserve(nheads * 4 * nslices) || p->fc.reserve(sizeof(FrameConsts)))
+        p->heads.reserve(nheads * wf_head_words() * 4 * nslices) || p->fc.reserve(sizeof(FrameConsts)))
         return SOLHIP_ERROR_OUT_OF_MEMORY;
     FrameConsts fc;
     std::memset(&fc, 0, sizeof(fc));
@@ -354,7 +357,7 @@ int trace_wavefront(Context *c, hipStream_t s, Program *p, Tlas *t, const ProgAr
             ws.sq = w.sq + qb;
             ws.scontrib = w.scontrib + qb;
             ws.counts = w.counts + 64u * j;
-            ws.heads = w.heads + nheads * j;
+            ws.heads = w.heads + nheads * wf_head_words() * j;
             ws.shade_blocks = (uint32_t)std::min<uint64_t>(((uint64_t)w.S * ws.nslots + w.block - 1) / w.block,
                                                            w.shade_blocks);
             hipStream_t ss = p->slice_stream[j];

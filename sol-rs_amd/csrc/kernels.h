@@ -142,7 +142,7 @@ struct WfArgs {
     QRay *sq;                         // shadow queue
     float4 *scontrib;                 // shadow-ray contributions, by shadow slot
     uint32_t *counts;                 // [0], [1]: extension queue sizes, [2]: shadow queue size
-    uint32_t *heads;                  // dynamic-fetch heads, 2 per bounce
+    uint32_t *heads;                  // fetch heads, 2 per bounce, wf_head_words() words each
     const LightTri *lights;
     uint32_t nlights;
     NrcArgs nrc;
@@ -188,6 +188,8 @@ int launch_pathtrace_passes(hipStream_t s, const WfArgs &a, Timer *timer, const 
 int launch_pathtrace_finalize(hipStream_t s, const WfArgs &a, Timer *timer);
 // threads per CU that the traversal kernels' register budget keeps resident
 uint32_t wf_threads_per_cu();
+// words of WfArgs::heads per logical fetch head (its partitions' heads)
+uint32_t wf_head_words();
 // writes the frame constants through a kernel argument (no host buffer to keep alive)
 int launch_set_frame(hipStream_t s, FrameConsts *dst, const FrameConsts &value);
 

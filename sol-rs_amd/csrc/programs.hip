@@ -17,6 +17,7 @@
 #include "shading.h"
 #include "traverse.h"
 #include "traverse_step.h"
+#include "fetch.h"
 
 namespace solhip {
 
@@ -29,21 +30,17 @@ namespace solhip {
 // persistent AO rays: a wave takes new rays for its idle lanes once at least
 // this many are idle (0: one ray per lane, no refill)
 #ifndef SOLHIP_AO_REFILL
-#define SOLHIP_AO_REFILL 40
+#define SOLHIP_AO_REFILL 16
 #endif
 // 1: k_ao_primary writes every AO ray (32 B) and the persistent kernel's
 // refill only loads it; 0: the refill samples the direction itself
 #ifndef SOLHIP_AO_RAYBUF
-#define SOLHIP_AO_RAYBUF 1
+#define SOLHIP_AO_RAYBUF 0
 #endif
-// the AO rays are split into this many contiguous partitions, each with its
-// own fetch head (128 B apart); a wave starts on partition wave % PARTS and
-// moves to the next one when its partition is used up, so refills spread
-// their atomics over PARTS addresses instead of one
+// fetch partitions of the persistent AO kernel (fetch.h)
 #ifndef SOLHIP_AO_PARTS
 #define SOLHIP_AO_PARTS 64
 #endif
-constexpr uint32_t kHeadStride = 32;   // words between heads
 
 // ---------------------------------------------------------------------------
 // plain ray-traced AO (wavefront)
@@ -199,32 +196,16 @@ __global__ void __launch_bounds__(kProgBlock) SOLHIP_AO_RAYS_OCCUPANCY k_ao_rays
     TravCounts tc{0, 0, 0, false};
     uint32_t rays = 0;
     TravState st;
-    bool has = false, exhausted = false;
+    bool has = false;
     uint32_t rid = 0, pix = 0;
-    // ray partitions [total * k / P, total * (k + 1) / P)
-    const uint32_t P = A.aoheads ? (uint32_t)SOLHIP_AO_PARTS : 1u;
-    uint32_t part = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) % P, moved = 0;
-    uint32_t *head = A.aoheads ? A.aoheads + part * kHeadStride : A.qcount + 1;
-    uint32_t lo = (uint32_t)((uint64_t)total * part / P), hi = (uint32_t)((uint64_t)total * (part + 1) / P);
+    PartFetch pf;
+    pf.init(A.aoheads, SOLHIP_AO_PARTS, total);
     while (true) {
         const uint64_t idle = __ballot(!has);
         const uint32_t nidle = (uint32_t)__popcll(idle);
-        if (!exhausted && nidle >= (uint32_t)SOLHIP_AO_REFILL) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(head, nidle);
-            base = lo + __shfl(base, 0);
-            const uint32_t end = hi;
-            if (base + nidle >= end) {
-                // this partition is used up: the next one, until all were seen
-                if (++moved >= P) {
-                    exhausted = true;
-                } else {
-                    part = part + 1 == P ? 0 : part + 1;
-                    head = A.aoheads + part * kHeadStride;
-                    lo = (uint32_t)((uint64_t)total * part / P);
-                    hi = (uint32_t)((uint64_t)total * (part + 1) / P);
-                }
-            }
+        if (!pf.exhausted && nidle >= (uint32_t)SOLHIP_AO_REFILL) {
+            uint32_t end;
+            const uint32_t base = pf.claim(nidle, end);
             if (!has) {
                 const uint32_t my = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
                 if (my < end) {
@@ -246,7 +227,7 @@ __global__ void __launch_bounds__(kProgBlock) SOLHIP_AO_RAYS_OCCUPANCY k_ao_rays
                 }
             }
         }
-        if (__ballot(has) == 0 && exhausted) break;
+        if (__ballot(has) == 0 && pf.exhausted) break;
         if (has && trav_step<ANY, COUNT>(st, A.sv, 0xFFu, stack + threadIdx.x, kProgBlock, tc)) {
             const Hit h = trav_result(st);
             A.occl[rid] = h.inst != SOLHIP_MISS ? 1u : 0u;
@@ -295,7 +276,7 @@ static int launch_status(const char *what)
         if (timer) (timer)->end(s, id);                                                       \
     } while (0)
 
-uint64_t ao_head_bytes() { return SOLHIP_AO_PARTS > 1 && SOLHIP_AO_REFILL ? SOLHIP_AO_PARTS * kHeadStride * 4ull : 0; }
+uint64_t ao_head_bytes() { return SOLHIP_AO_REFILL ? SOLHIP_AO_PARTS * kHeadStride * 4ull : 0; }
 
 uint64_t ao_ray_bytes(uint64_t npix, uint32_t n_ao)
 {
@@ -306,7 +287,7 @@ int launch_ao(hipStream_t s, const ProgArgs &a, Timer *timer)
 {
     const uint32_t tiles = tile_count(a.W, a.H);
     SOLHIP_LAUNCH(k_ao_primary, SOLHIP_KERNEL_AO_PRIMARY, tiles, a, s, timer);
-    if (SOLHIP_AO_REFILL == 0 || !a.occl) {
+    if (SOLHIP_AO_REFILL == 0 || !a.occl || !a.aoheads) {
         const uint64_t lanes = (uint64_t)a.W * a.H * a.n_ao;
         const uint32_t blocks = (uint32_t)((lanes + kProgBlock - 1) / kProgBlock);
         SOLHIP_LAUNCH(k_ao_rays, SOLHIP_KERNEL_AO_RAYS, blocks, a, s, timer);
@@ -339,8 +320,6 @@ template <bool COUNT>
 __global__ void __launch_bounds__(kProgBlock) k_ao_glsl(ProgArgs A)
 {
     extern __shared__ uint2 stack[];
-    if (A.aoheads && blockIdx.x == 0)
-        for (uint32_t i = threadIdx.x; i < SOLHIP_AO_PARTS; i += blockDim.x) A.aoheads[i * kHeadStride] = 0;
     uint32_t x, y;
     const bool active = tile_pixel(A.W, A.H, x, y);
     TravCounts tc{0, 0, 0, false};
@@ -417,8 +396,6 @@ template <bool COUNT>
 __global__ void __launch_bounds__(kProgBlock) k_pathtrace_glsl(ProgArgs A)
 {
     extern __shared__ uint2 stack[];
-    if (A.aoheads && blockIdx.x == 0)
-        for (uint32_t i = threadIdx.x; i < SOLHIP_AO_PARTS; i += blockDim.x) A.aoheads[i * kHeadStride] = 0;
     uint32_t x, y;
     const bool active = tile_pixel(A.W, A.H, x, y);
     TravCounts tc{0, 0, 0, false};
@@ -530,8 +507,6 @@ template <bool COUNT>
 __global__ void __launch_bounds__(kProgBlock) k_debug(ProgArgs A)
 {
     extern __shared__ uint2 stack[];
-    if (A.aoheads && blockIdx.x == 0)
-        for (uint32_t i = threadIdx.x; i < SOLHIP_AO_PARTS; i += blockDim.x) A.aoheads[i * kHeadStride] = 0;
     uint32_t x, y;
     const bool active = tile_pixel(A.W, A.H, x, y);
     TravCounts tc{0, 0, 0, false};

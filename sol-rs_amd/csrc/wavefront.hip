@@ -30,6 +30,7 @@
 #include "shading.h"
 #include "traverse.h"
 #include "traverse_step.h"
+#include "fetch.h"
 
 namespace solhip {
 
@@ -66,6 +67,12 @@ __device__ __forceinline__ float ray_tmin(const WfArgs &A)
     return fmaxf(1.0f, sqrtf(o[0] * o[0] + o[1] * o[1] + o[2] * o[2])) * 1e-3f;
 }
 
+// fetch partitions of the extend and shadow kernels (fetch.h)
+#ifndef SOLHIP_WF_PARTS
+#define SOLHIP_WF_PARTS 64
+#endif
+constexpr uint32_t kWfParts = SOLHIP_WF_PARTS;
+
 // samples traced in pass `pass` (the last pass may hold fewer than S)
 __device__ __forceinline__ uint32_t pass_samples(const WfArgs &A, uint32_t pass)
 {
@@ -83,7 +90,7 @@ __global__ void k_pt_generate(WfArgs A, uint32_t pass)
         A.counts[1] = 0;
         A.counts[2] = 0;
     }
-    if (e < 2 * (A.max_depth + 1)) A.heads[e] = 0;
+    if (e < 2 * (A.max_depth + 1) * kWfParts) A.heads[e * kHeadStride] = 0;
     if (e >= n) return;
     const uint32_t sl = e / A.nslots, slot = e - sl * A.nslots;
     uint32_t x, y;
@@ -185,7 +192,7 @@ __device__ __forceinline__ void queue_result(const WfArgs &A, uint32_t i, uint32
 // while the rest of the wave is still traversing.  sv.stack_depth includes
 // the two extra entries the step traversal may hold.
 template <bool ANY, bool COUNT>
-__device__ __forceinline__ void trace_queue(const WfArgs &A, const QRay *q, uint32_t cnt, uint32_t *head, int kernel)
+__device__ __forceinline__ void trace_queue(const WfArgs &A, const QRay *q, uint32_t cnt, uint32_t *heads, int kernel)
 {
     extern __shared__ uint2 stack[];
     const uint32_t lane = threadIdx.x & 63u;
@@ -194,19 +201,19 @@ __device__ __forceinline__ void trace_queue(const WfArgs &A, const QRay *q, uint
     uint32_t rays = 0;
 #if SOLHIP_WF_REFILL
     TravState st;
-    bool has = false, exhausted = false;
+    bool has = false;
     uint32_t idx = 0, path = 0;
+    PartFetch pf;
+    pf.init(heads, kWfParts, cnt);
     while (true) {
         const uint64_t idle = __ballot(!has);
         const uint32_t nidle = (uint32_t)__popcll(idle);
-        if (!exhausted && nidle >= (uint32_t)SOLHIP_WF_REFILL) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(head, nidle);
-            base = __shfl(base, 0);
-            exhausted = base + nidle >= cnt;
+        if (!pf.exhausted && nidle >= (uint32_t)SOLHIP_WF_REFILL) {
+            uint32_t end;
+            const uint32_t base = pf.claim(nidle, end);
             if (!has) {
                 const uint32_t my = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-                if (my < cnt) {
+                if (my < end) {
                     const QRay r = q[my];
                     if (r.path == kNoPath) {
                         if (!ANY) {
@@ -225,7 +232,7 @@ __device__ __forceinline__ void trace_queue(const WfArgs &A, const QRay *q, uint
                 }
             }
         }
-        if (__ballot(has) == 0) break;
+        if (__ballot(has) == 0 && pf.exhausted) break;
         if (has && trav_step<ANY, COUNT>(st, A.sv, 0xFFu, stack + threadIdx.x, A.block, tc)) {
             queue_result<ANY>(A, idx, path, trav_result(st));
             ++rays;
@@ -235,7 +242,7 @@ __device__ __forceinline__ void trace_queue(const WfArgs &A, const QRay *q, uint
 #else
     for (;;) {
         uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(head, 64u);
+        if (lane == 0) base = atomicAdd(heads, 64u);
         base = __shfl(base, 0);
         if (base >= cnt) break;
         const uint32_t i = base + lane;
@@ -265,13 +272,15 @@ __global__ void SOLHIP_WF_OCCUPANCY k_pt_extend(WfArgs A, uint32_t b)
         A.counts[(b + 1) & 1] = 0;
         A.counts[2] = 0;
     }
-    trace_queue<false, COUNT>(A, A.q[b & 1], A.counts[b & 1], A.heads + 2 * b, SOLHIP_KERNEL_PT_EXTEND);
+    trace_queue<false, COUNT>(A, A.q[b & 1], A.counts[b & 1], A.heads + 2 * b * kWfParts * kHeadStride,
+                              SOLHIP_KERNEL_PT_EXTEND);
 }
 
 template <bool COUNT>
 __global__ void SOLHIP_WF_OCCUPANCY k_pt_shadow(WfArgs A, uint32_t b)
 {
-    trace_queue<true, COUNT>(A, A.sq, A.counts[2], A.heads + 2 * b + 1, SOLHIP_KERNEL_PT_SHADOW);
+    trace_queue<true, COUNT>(A, A.sq, A.counts[2], A.heads + (2 * b + 1) * kWfParts * kHeadStride,
+                             SOLHIP_KERNEL_PT_SHADOW);
 }
 
 __global__ void k_pt_shade(WfArgs A, uint32_t b)
@@ -650,6 +659,8 @@ int launch_status(const char *what)
 
 uint32_t wf_threads_per_cu() { return 4u * SOLHIP_WF_WAVES * 64u; }
 
+uint32_t wf_head_words() { return kWfParts * kHeadStride; }
+
 int launch_lights_count(hipStream_t s, const DevTables &tb, const uint32_t *inst_tris, uint32_t n_inst,
                         uint32_t *offsets, Timer *timer)
 {
@@ -702,7 +713,7 @@ int launch_pathtrace_passes(hipStream_t s, const WfArgs &a0, Timer *timer, const
     const uint32_t blk = a.block;
     const size_t lds = (size_t)a.sv.stack_depth * sizeof(uint2) * blk;
     const uint32_t nheads = 2 * (a.max_depth + 1);
-    const uint32_t gen_blocks = (std::max(a.S * a.nslots, nheads) + blk - 1) / blk;
+    const uint32_t gen_blocks = (std::max(a.S * a.nslots, nheads * kWfParts) + blk - 1) / blk;
     const uint32_t fold_blocks = (a.nslots + blk - 1) / blk;
     const uint32_t passes = (a.spp + a.S - 1) / a.S;
     const bool shadows = a.nee && a.nlights > 0;
