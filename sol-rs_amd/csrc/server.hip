@@ -31,6 +31,7 @@
 #include "shading.h"
 #include "traverse.h"
 #include "traverse_step.h"
+#include "fetch.h"
 
 namespace solhip {
 
@@ -43,6 +44,8 @@ constexpr uint32_t kSrvBlock = 64;
 #define SOLHIP_SRV_WAVES 5
 #endif
 #define SOLHIP_SRV_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SOLHIP_SRV_WAVES)))
+// fetch partitions of the extend and shadow kernels (fetch.h)
+constexpr uint32_t kSrvParts = 64;
 constexpr uint32_t kMaxJobs = 256;   // job index lives in 8 bits of the slot meta word
 
 // per-job constants (device array, one entry per running-job index)
@@ -71,7 +74,7 @@ struct SrvArgs {
     float4 *scontrib;
     uint32_t *retire;
     uint32_t *counts;     // [0], [1] extension queues, [2] shadow queue, [3] retire list
-    uint32_t *heads;      // [0] extend, [1] shadow
+    uint32_t *heads;      // partitioned fetch heads: extend, then shadow (kSrvParts each)
     uint32_t *done;       // finished pixels per job index
     Counters *counters;
 };
@@ -154,7 +157,7 @@ __global__ void k_srv_admit(SrvArgs A, uint32_t j, uint32_t cur, uint32_t nslots
 // through the resumable traversal; a lane takes new slots once at least 16
 // lanes of its wave are idle
 template <bool ANY>
-__device__ __forceinline__ void srv_trace(const SrvArgs &A, const QRay *q, uint32_t cnt, uint32_t *head, int kernel)
+__device__ __forceinline__ void srv_trace(const SrvArgs &A, const QRay *q, uint32_t cnt, uint32_t *heads, int kernel)
 {
     extern __shared__ uint2 stack[];
     const uint32_t lane = threadIdx.x & 63u;
@@ -162,19 +165,19 @@ __device__ __forceinline__ void srv_trace(const SrvArgs &A, const QRay *q, uint3
     uint32_t rays = 0;
     TravState st;
     SceneView sv;
-    bool has = false, exhausted = false;
+    bool has = false;
     uint32_t idx = 0, path = 0;
+    PartFetch pf;
+    pf.init(heads, kSrvParts, cnt);
     while (true) {
         const uint64_t idle = __ballot(!has);
         const uint32_t nidle = (uint32_t)__popcll(idle);
-        if (!exhausted && nidle >= 16u) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(head, nidle);
-            base = __shfl(base, 0);
-            exhausted = base + nidle >= cnt;
+        if (!pf.exhausted && nidle >= 16u) {
+            uint32_t end;
+            const uint32_t base = pf.claim(nidle, end);
             if (!has) {
                 const uint32_t my = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-                if (my < cnt) {
+                if (my < end) {
                     const QRay r = q[my];
                     const JobDev &J = A.jobs[A.pmeta[r.path] & 0xFFu];
                     sv = J.sv;
@@ -186,7 +189,7 @@ __device__ __forceinline__ void srv_trace(const SrvArgs &A, const QRay *q, uint3
                 }
             }
         }
-        if (__ballot(has) == 0) break;
+        if (__ballot(has) == 0 && pf.exhausted) break;
         if (has && trav_step<ANY, false>(st, sv, 0xFFu, stack + threadIdx.x, kSrvBlock, tc)) {
             const Hit h = trav_result(st);
             ++rays;
@@ -221,12 +224,12 @@ __global__ void __launch_bounds__(kSrvBlock) SOLHIP_SRV_OCCUPANCY k_srv_extend(S
         A.counts[2] = 0;
         A.counts[3] = 0;
     }
-    srv_trace<false>(A, A.q[cur], A.counts[cur], A.heads + 0, SOLHIP_KERNEL_SRV_EXTEND);
+    srv_trace<false>(A, A.q[cur], A.counts[cur], A.heads, SOLHIP_KERNEL_SRV_EXTEND);
 }
 
 __global__ void __launch_bounds__(kSrvBlock) SOLHIP_SRV_OCCUPANCY k_srv_shadow(SrvArgs A)
 {
-    srv_trace<true>(A, A.sq, A.counts[2], A.heads + 1, SOLHIP_KERNEL_SRV_SHADOW);
+    srv_trace<true>(A, A.sq, A.counts[2], A.heads + kSrvParts * kHeadStride, SOLHIP_KERNEL_SRV_SHADOW);
 }
 
 __global__ void __launch_bounds__(kSrvBlock) k_srv_shade(SrvArgs A, uint32_t cur)
@@ -272,10 +275,9 @@ __global__ void __launch_bounds__(kSrvBlock) k_srv_shade(SrvArgs A, uint32_t cur
 
 __global__ void __launch_bounds__(kSrvBlock) k_srv_retire(SrvArgs A, uint32_t cur)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (blockIdx.x == 0) {
         // this step's extend and shadow are complete
-        A.heads[0] = 0;
-        A.heads[1] = 0;
+        for (uint32_t k = threadIdx.x; k < 2 * kSrvParts; k += blockDim.x) A.heads[k * kHeadStride] = 0;
     }
     const uint32_t cnt = A.counts[3];
     const uint32_t lane = threadIdx.x & 63u, nthreads = gridDim.x * blockDim.x;
@@ -588,7 +590,7 @@ int32_t solhip_server_create(solhip_ctx ctx, uint64_t hbm_budget, uint32_t max_j
                 sv->q0.reserve(n * sizeof(QRay)) || sv->q1.reserve(n * sizeof(QRay)) ||
                 sv->hits.reserve(n * sizeof(solhip_hit)) || sv->sq.reserve(n * sizeof(QRay)) ||
                 sv->scontrib.reserve(n * 16) || sv->retire.reserve(n * 4) || sv->counts.reserve(256) ||
-                sv->heads.reserve(256) || sv->done.reserve((uint64_t)max_jobs * 4);
+                sv->heads.reserve(2ull * kSrvParts * kHeadStride * 4) || sv->done.reserve((uint64_t)max_jobs * 4);
     if (!fail) fail = hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking) != hipSuccess ||
                       hipHostMalloc((void **)&sv->done_host, sizeof(uint32_t) * max_jobs, hipHostMallocDefault) !=
                           hipSuccess;
@@ -599,7 +601,7 @@ int32_t solhip_server_create(solhip_ctx ctx, uint64_t hbm_budget, uint32_t max_j
         return SOLHIP_ERROR_OUT_OF_MEMORY;
     }
     if (launch_fill(sv->stream, (uint32_t *)sv->counts.ptr, 64, 0u) ||
-        launch_fill(sv->stream, (uint32_t *)sv->heads.ptr, 64, 0u) ||
+        launch_fill(sv->stream, (uint32_t *)sv->heads.ptr, 2ull * kSrvParts * kHeadStride, 0u) ||
         launch_fill(sv->stream, (uint32_t *)sv->done.ptr, max_jobs, 0u) || hipStreamSynchronize(sv->stream)) {
         sv->release_all();
         delete sv;
