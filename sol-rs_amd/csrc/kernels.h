@@ -58,7 +58,7 @@ struct ProgArgs {
     uint32_t *qcount;       // [0] queue entries, [1] ray cursor of the persistent AO kernel
     uint8_t *occl;          // per AO ray: occluded (persistent AO kernel)
     QRay *aorays;           // per AO ray: origin, pixel, direction (written by k_ao_primary; nullptr: none)
-    uint32_t *aoheads;      // fetch heads of the persistent AO kernel's ray partitions, 128 B apart
+    uint32_t *aoheads;      // fetch heads of the persistent AO kernel's partitions, 128 B apart
     uint32_t ncu;
     // pathtrace
     uint32_t enable_sky, spp, max_depth;

@@ -45,58 +45,45 @@ namespace solhip {
 // ---------------------------------------------------------------------------
 // plain ray-traced AO (wavefront)
 // ---------------------------------------------------------------------------
-template <bool COUNT>
-__global__ void __launch_bounds__(kProgBlock) k_ao_primary(ProgArgs A)
+// the primary hit of pixel pix (ray direction d, RNG after the jitter): a hit
+// becomes an AO queue entry (returns true); a miss has nothing to occlude and
+// takes the value 1 at once
+__device__ __forceinline__ bool ao_primary_result(const ProgArgs &A, uint32_t pix, uint32_t rng, V3 d, const Hit &h,
+                                                  AoEntry &e)
 {
-    extern __shared__ uint2 stack[];
-    if (A.aoheads && blockIdx.x == 0)
-        for (uint32_t i = threadIdx.x; i < SOLHIP_AO_PARTS; i += blockDim.x) A.aoheads[i * kHeadStride] = 0;
-    uint32_t x, y;
-    const bool active = tile_pixel(A.W, A.H, x, y);
-    TravCounts tc{0, 0, 0, false};
-    bool hit = false;
-    AoEntry e;
-    if (active) {
-        const uint32_t pix = y * A.W + x;
-        uint32_t rng = tea(x + y * A.W, A.frame);
-        const float jx = next_rand(rng);
-        const float jy = next_rand(rng);
-        V3 o, d;
-        camera_ray(A.cam.view_inverse, A.cam.projection_inverse, x, y, A.W, A.H, jx, jy, o, d);
-        const float tmin = fmaxf(1.0f, length3(o)) * 1e-3f;
-        const Hit h = trace_ray<false, COUNT>(A.sv, o, d, tmin, 10000.0f, 0xFFu, stack + threadIdx.x, kProgBlock, tc);
-        if (A.hits_out) store_hit(A.hits_out + pix, h);
-        if (h.inst != SOLHIP_MISS) {
-            const Surf sf = fetch_surface(A.tb, h.inst, h.prim, h.u, h.v);
-            const float sg = sgn(dot(neg(d), sf.nrm));
-            e.pixel = pix;
-            e.px = sf.pos.x;
-            e.py = sf.pos.y;
-            e.pz = sf.pos.z;
-            e.nx = sf.nrm.x * sg;
-            e.ny = sf.nrm.y * sg;
-            e.nz = sf.nrm.z * sg;
-            e.rng = rng;
-            hit = true;
-        } else {
-            // nothing to occlude: value 1
-            float4 *px = A.accum + pix;
-            const float4 old = *px;
-            *px = make_float4(mixf(old.x, 1.0f, A.alpha), mixf(old.y, 1.0f, A.alpha), mixf(old.z, 1.0f, A.alpha),
-                              1.0f);
-            if (A.hits_out) {
-                solhip_hit m;
-                m.t = -1.0f;
-                m.u = 0.0f;
-                m.v = 0.0f;
-                m.inst = SOLHIP_MISS;
-                m.prim = SOLHIP_MISS;
-                solhip_hit *dst = A.hits_out + (uint64_t)A.W * A.H + (uint64_t)pix * A.n_ao;
-                for (uint32_t k = 0; k < A.n_ao; ++k) dst[k] = m;
-            }
-        }
+    if (A.hits_out) store_hit(A.hits_out + pix, h);
+    if (h.inst != SOLHIP_MISS) {
+        const Surf sf = fetch_surface(A.tb, h.inst, h.prim, h.u, h.v);
+        const float sg = sgn(dot(neg(d), sf.nrm));
+        e.pixel = pix;
+        e.px = sf.pos.x;
+        e.py = sf.pos.y;
+        e.pz = sf.pos.z;
+        e.nx = sf.nrm.x * sg;
+        e.ny = sf.nrm.y * sg;
+        e.nz = sf.nrm.z * sg;
+        e.rng = rng;
+        return true;
     }
-    // compact the hits into the AO queue: one atomic per wave
+    float4 *px = A.accum + pix;
+    const float4 old = *px;
+    *px = make_float4(mixf(old.x, 1.0f, A.alpha), mixf(old.y, 1.0f, A.alpha), mixf(old.z, 1.0f, A.alpha), 1.0f);
+    if (A.hits_out) {
+        solhip_hit m;
+        m.t = -1.0f;
+        m.u = 0.0f;
+        m.v = 0.0f;
+        m.inst = SOLHIP_MISS;
+        m.prim = SOLHIP_MISS;
+        solhip_hit *dst = A.hits_out + (uint64_t)A.W * A.H + (uint64_t)pix * A.n_ao;
+        for (uint32_t k = 0; k < A.n_ao; ++k) dst[k] = m;
+    }
+    return false;
+}
+
+// the wave's hits into the AO queue with one atomic; every lane must call it
+__device__ __forceinline__ void ao_queue_append(const ProgArgs &A, bool hit, const AoEntry &e)
+{
     const uint64_t mask = __ballot(hit);
     if (mask) {
         const uint32_t lane = threadIdx.x & 63;
@@ -107,30 +94,38 @@ __global__ void __launch_bounds__(kProgBlock) k_ao_primary(ProgArgs A)
             const uint32_t slot = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
             A.queue[slot] = e;
-            if (A.aorays) {
-                // the hit's AO rays in order: two numbers each from the
-                // pixel's stream (ao.rchit's cosine lobe around the normal)
-                uint32_t rng = e.rng;
-                const V3 n = v3(e.nx, e.ny, e.nz);
-                QRay *dst = A.aorays + (uint64_t)slot * A.n_ao;
-                for (uint32_t k = 0; k < A.n_ao; ++k) {
-                    const float xi0 = next_rand(rng);
-                    const float xi1 = next_rand(rng);
-                    const V3 dir = sample_cosine(n, xi0, xi1);
-                    QRay r;
-                    r.ox = e.px;
-                    r.oy = e.py;
-                    r.oz = e.pz;
-                    r.path = e.pixel;
-                    r.dx = dir.x;
-                    r.dy = dir.y;
-                    r.dz = dir.z;
-                    r.tmax = A.ao_tmax;
-                    dst[k] = r;
-                }
-            }
         }
     }
+}
+
+// camera ray of pixel (x, y) with the jitter of ao.rgen:49-54; rng is the
+// pixel's stream after the two jitter numbers
+__device__ __forceinline__ void ao_camera(const ProgArgs &A, uint32_t x, uint32_t y, uint32_t &rng, V3 &o, V3 &d)
+{
+    rng = tea(x + y * A.W, A.frame);
+    const float jx = next_rand(rng);
+    const float jy = next_rand(rng);
+    camera_ray(A.cam.view_inverse, A.cam.projection_inverse, x, y, A.W, A.H, jx, jy, o, d);
+}
+
+template <bool COUNT>
+__global__ void __launch_bounds__(kProgBlock) k_ao_primary(ProgArgs A)
+{
+    extern __shared__ uint2 stack[];
+    uint32_t x, y;
+    const bool active = tile_pixel(A.W, A.H, x, y);
+    TravCounts tc{0, 0, 0, false};
+    bool hit = false;
+    AoEntry e;
+    if (active) {
+        uint32_t rng;
+        V3 o, d;
+        ao_camera(A, x, y, rng, o, d);
+        const float tmin = fmaxf(1.0f, length3(o)) * 1e-3f;
+        const Hit h = trace_ray<false, COUNT>(A.sv, o, d, tmin, 10000.0f, 0xFFu, stack + threadIdx.x, kProgBlock, tc);
+        hit = ao_primary_result(A, y * A.W + x, rng, d, h, e);
+    }
+    ao_queue_append(A, hit, e);
     count_rays<COUNT>(A.counters + SOLHIP_KERNEL_AO_PRIMARY, active ? 1u : 0u, tc);
 }
 
@@ -276,18 +271,18 @@ static int launch_status(const char *what)
         if (timer) (timer)->end(s, id);                                                       \
     } while (0)
 
-uint64_t ao_head_bytes() { return SOLHIP_AO_REFILL ? SOLHIP_AO_PARTS * kHeadStride * 4ull : 0; }
+uint64_t ao_head_bytes() { return SOLHIP_AO_REFILL ? (uint64_t)SOLHIP_AO_PARTS * kHeadStride * 4 : 0; }
 
 uint64_t ao_ray_bytes(uint64_t npix, uint32_t n_ao)
 {
-    return SOLHIP_AO_RAYBUF && SOLHIP_AO_REFILL ? npix * n_ao * sizeof(QRay) : 0;
+    constexpr bool kRayBuf = SOLHIP_AO_RAYBUF != 0 && SOLHIP_AO_REFILL != 0;
+    return kRayBuf ? npix * n_ao * sizeof(QRay) : 0;
 }
 
 int launch_ao(hipStream_t s, const ProgArgs &a, Timer *timer)
 {
-    const uint32_t tiles = tile_count(a.W, a.H);
-    SOLHIP_LAUNCH(k_ao_primary, SOLHIP_KERNEL_AO_PRIMARY, tiles, a, s, timer);
     if (SOLHIP_AO_REFILL == 0 || !a.occl || !a.aoheads) {
+        SOLHIP_LAUNCH(k_ao_primary, SOLHIP_KERNEL_AO_PRIMARY, tile_count(a.W, a.H), a, s, timer);
         const uint64_t lanes = (uint64_t)a.W * a.H * a.n_ao;
         const uint32_t blocks = (uint32_t)((lanes + kProgBlock - 1) / kProgBlock);
         SOLHIP_LAUNCH(k_ao_rays, SOLHIP_KERNEL_AO_RAYS, blocks, a, s, timer);
@@ -299,6 +294,10 @@ int launch_ao(hipStream_t s, const ProgArgs &a, Timer *timer)
     p.sv.stack_depth = a.sv.stack_depth + 2;
     const size_t lds = (size_t)p.sv.stack_depth * sizeof(uint2) * kProgBlock;
     const uint32_t blocks = a.ncu * 4 * SOLHIP_AO_WAVES;
+    // the fetch heads start at zero (a fill kernel: no memset node in a captured graph)
+    int rc = launch_fill(s, a.aoheads, (uint64_t)SOLHIP_AO_PARTS * kHeadStride, 0u);
+    if (rc) return rc;
+    SOLHIP_LAUNCH(k_ao_primary, SOLHIP_KERNEL_AO_PRIMARY, tile_count(a.W, a.H), a, s, timer);
     if (timer) timer->begin(s, SOLHIP_KERNEL_AO_RAYS);
     if (a.ao_any) {
         if (a.count) hipLaunchKernelGGL((k_ao_rays_persist<true, true>), dim3(blocks), dim3(kProgBlock), lds, s, p);
