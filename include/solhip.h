@@ -307,9 +307,10 @@ int32_t solhip_kernel_stats_get(solhip_ctx ctx, uint32_t kernel, solhip_kernel_s
 /* ---- launch-shape autotuner for SOLHIP_PROG_PATHTRACE (no counterpart in
  * the reference, whose dispatch shape is the driver's) --------------------
  * Times `frames` frames of every candidate launch shape (wf_block in
- * {64, 128, 256} x three persistent grid sizes x wf_graph in {0, 1}) for the
- * estimator settings in *spec on this scene and image size, and returns *spec
- * with the fastest shape in *best (and its frame time).  Results persist in
+ * {64, 128, 256} x three persistent grid sizes x wf_graph in {0, 1}, then,
+ * with sample_streams, wf_slices in {1, 2, 4, 8} for the best of those) for
+ * the estimator settings in *spec on this scene and image size, and returns
+ * *spec with the fastest shape in *best (and its frame time).  Results persist in
  * cache_path (text, one line per workload: device, CUs, size, estimator
  * settings, instance and triangle counts); a cached workload is answered
  * without timing.  The timed frames render into images->accum / out_rgba8.

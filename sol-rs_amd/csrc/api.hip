@@ -983,22 +983,25 @@ int32_t solhip_pathtrace_autotune(solhip_ctx ctx, solhip_stream st, solhip_tlas 
         if (b) tris += b->bvh.prim_count;
     }
     char key[256];
-    std::snprintf(key, sizeof(key), "gfx950 cu=%u %ux%u spp=%u depth=%u nee=%u rr=%u sky=%u inst=%u blas_tris=%llu",
+    std::snprintf(key, sizeof(key),
+                  "gfx950 cu=%u %ux%u spp=%u depth=%u nee=%u rr=%u sky=%u streams=%u per_pass=%u inst=%u blas_tris=%llu",
                   c->ncu, W, H, spec->spp_per_frame, spec->max_depth, spec->nee, spec->rr_start, spec->enable_sky,
-                  t->n_inst, (unsigned long long)tris);
+                  spec->sample_streams, spec->wf_samples_per_pass, t->n_inst, (unsigned long long)tris);
     if (cache_path) {
         if (FILE *f = std::fopen(cache_path, "r")) {
             char line[512];
             while (std::fgets(line, sizeof(line), f)) {
                 const size_t kl = std::strlen(key);
-                unsigned blk = 0, pcu = 0, gr = 0;
+                unsigned blk = 0, pcu = 0, gr = 0, sl = 0;
                 float ms = 0.0f;
                 if (std::strncmp(line, key, kl) == 0 && line[kl] == ' ' &&
-                    std::sscanf(line + kl, " -> block=%u per_cu=%u graph=%u ms=%f", &blk, &pcu, &gr, &ms) == 4) {
+                    std::sscanf(line + kl, " -> block=%u per_cu=%u graph=%u slices=%u ms=%f", &blk, &pcu, &gr, &sl,
+                                &ms) == 5) {
                     *best = *spec;
                     best->wf_block = blk;
                     best->wf_blocks_per_cu = pcu;
                     best->wf_graph = gr;
+                    best->wf_slices = sl;
                     if (best_ms) *best_ms = ms;
                     std::fclose(f);
                     return SOLHIP_OK;
@@ -1015,39 +1018,51 @@ int32_t solhip_pathtrace_autotune(solhip_ctx ctx, solhip_stream st, solhip_tlas 
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = SOLHIP_OK;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = SOLHIP_ERROR_DEVICE;
+    // one candidate: an untimed frame (allocation, capture), then the timed ones
+    auto time_candidate = [&](const solhip_spec &cand) {
+        solhip_program ph = 0;
+        rc = solhip_program_create(ctx, SOLHIP_PROG_PATHTRACE, &cand, &ph);
+        if (rc) return;
+        solhip_scene_uniforms uf = *u;
+        for (uint32_t f = 0; f <= frames && !rc; ++f) {
+            if (f == 1) (void)hipEventRecord(e0, s);
+            uf.frame[2] = u->frame[2] + f;
+            rc = solhip_trace(ctx, st, ph, th, tables, &uf, (int32_t)uf.frame[2], images, W, H);
+        }
+        if (!rc) {
+            (void)hipEventRecord(e1, s);
+            (void)hipEventSynchronize(e1);
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            ms /= (float)frames;
+            if (ms < best_t) {
+                best_t = ms;
+                winner = cand;
+            }
+        }
+        (void)solhip_program_destroy(ctx, ph);
+    };
+    // launch shape, then (per-sample streams only) the image slices of the winner
     for (uint32_t blk : {64u, 128u, 256u}) {
         if ((uint64_t)(depth + 2) * 8 * blk > 65536) continue;
         const uint32_t base = std::max(1u, wf_threads_per_cu() / blk);
-        for (uint32_t pcu : {base / 2 ? base / 2 : 1u, base, 2 * base}) {
+        for (uint32_t pcu : {base / 2 ? base / 2 : 1u, base, 2 * base})
             for (uint32_t gr : {0u, 1u}) {
                 if (rc) break;
                 solhip_spec cand = *spec;
                 cand.wf_block = blk;
                 cand.wf_blocks_per_cu = pcu;
                 cand.wf_graph = gr;
-                solhip_program ph = 0;
-                rc = solhip_program_create(ctx, SOLHIP_PROG_PATHTRACE, &cand, &ph);
-                if (rc) break;
-                solhip_scene_uniforms uf = *u;
-                // one untimed frame (allocation, capture), then the timed ones
-                for (uint32_t f = 0; f <= frames && !rc; ++f) {
-                    if (f == 1) (void)hipEventRecord(e0, s);
-                    uf.frame[2] = u->frame[2] + f;
-                    rc = solhip_trace(ctx, st, ph, th, tables, &uf, (int32_t)uf.frame[2], images, W, H);
-                }
-                if (!rc) {
-                    (void)hipEventRecord(e1, s);
-                    (void)hipEventSynchronize(e1);
-                    float ms = 0.0f;
-                    (void)hipEventElapsedTime(&ms, e0, e1);
-                    ms /= (float)frames;
-                    if (ms < best_t) {
-                        best_t = ms;
-                        winner = cand;
-                    }
-                }
-                (void)solhip_program_destroy(ctx, ph);
+                time_candidate(cand);
             }
+    }
+    if (spec->sample_streams && !spec->nrc) {
+        const solhip_spec shape = winner;
+        for (uint32_t sl : {1u, 2u, 4u, 8u}) {
+            if (rc) break;
+            solhip_spec cand = shape;
+            cand.wf_slices = sl;
+            time_candidate(cand);
         }
     }
     if (e0) (void)hipEventDestroy(e0);
@@ -1058,8 +1073,8 @@ int32_t solhip_pathtrace_autotune(solhip_ctx ctx, solhip_stream st, solhip_tlas 
     if (best_ms) *best_ms = best_t;
     if (cache_path) {
         if (FILE *f = std::fopen(cache_path, "a")) {
-            std::fprintf(f, "%s -> block=%u per_cu=%u graph=%u ms=%.4f\n", key, winner.wf_block, winner.wf_blocks_per_cu,
-                         winner.wf_graph, best_t);
+            std::fprintf(f, "%s -> block=%u per_cu=%u graph=%u slices=%u ms=%.4f\n", key, winner.wf_block,
+                         winner.wf_blocks_per_cu, winner.wf_graph, winner.wf_slices, best_t);
             std::fclose(f);
         }
     }

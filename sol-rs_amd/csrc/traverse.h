@@ -176,9 +176,27 @@ SOLHIP_HD uint32_t to_visit_order(uint32_t m, uint32_t oct)
     return m;
 }
 
+// count bits set from bit offset up: ((1 << count) - 1) << offset (count <=
+// 31).  BFM: on the device one v_bfm_b32 (the compiler turns the shifts into
+// a longer sequence).  The step traversal gains from it; trace_ray, whose
+// loop the compiler schedules around it less well, does not.
+template <bool BFM>
+SOLHIP_HD uint32_t bit_range(uint32_t count, uint32_t offset)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (BFM) {
+        uint32_t m;
+        asm("v_bfm_b32 %0, %1, %2" : "=v"(m) : "v"(count), "v"(offset));
+        return m;
+    }
+#endif
+    return ((1u << count) - 1u) << offset;
+}
+
 // Intersect the 8 child boxes of a node (branch-free).  Returns
 // internal-child hits in visit order (bit i <-> slot i ^ oct) and the mask of
 // leaf primitives (relative to prim_base) of the leaf children hit.
+template <bool BFM = false>
 SOLHIP_HD void intersect_node(const uint4 w0, const uint4 w1, const uint4 w2, const uint4 w3, const uint4 w4, V3 o,
                               V3 inv, uint32_t oct, float tlo, float thi, uint32_t &hits_int, uint32_t &prim_mask)
 {
@@ -215,7 +233,7 @@ SOLHIP_HD void intersect_node(const uint4 w0, const uint4 w1, const uint4 w2, co
         const bool hit = tn <= tf;
         // leaf slot: (count << 5) | offset; internal and empty slots: 0
         const uint32_t meta = ((s < 4 ? w1.z : w1.w) >> (8 * b)) & 0xFFu;
-        const uint32_t lm = ((1u << (meta >> 5)) - 1u) << (meta & 31u);
+        const uint32_t lm = bit_range<BFM>(meta >> 5, meta & 31u);
         hm |= hit ? (1u << s) : 0u;
         pm |= hit ? lm : 0u;
     }

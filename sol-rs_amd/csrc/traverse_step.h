@@ -198,7 +198,7 @@ SOLHIP_HD bool trav_step(TravState &s, const SceneView &sv, uint32_t cull, uint2
                     w4 = SOLHIP_G4(np)[4];
         if (COUNT) cnt.nodes++;
         const float thi = s.best.t + fabsf(s.best.t) * 0x1p-20f;
-        intersect_node(w0, w1, w2, w3, w4, s.o, s.inv, s.oct, s.tlo, thi, s.ng_hits, s.tg_mask);
+        intersect_node<true>(w0, w1, w2, w3, w4, s.o, s.inv, s.oct, s.tlo, thi, s.ng_hits, s.tg_mask);
         s.ng_base = w1.x;
         s.ng_imask = w0.w >> 24;
         s.tg_base = w1.y;

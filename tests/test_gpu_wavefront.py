@@ -268,3 +268,26 @@ def test_autotuner_picks_and_caches_a_shape(ctx, tmp_path):
     a0, _, _ = _render(ctx, gs, sd, solhip.PROG_PATHTRACE, spec, W, H, 1)
     a1, _, _ = _render(ctx, gs, sd, solhip.PROG_PATHTRACE, best, W, H, 1)
     assert (_bits(a0) != _bits(a1)).sum() == 0
+
+
+def test_autotuner_picks_slices_with_sample_streams(ctx, tmp_path):
+    """with per-sample streams the tuner also times image slices; its pick
+    renders the same image as the default shape"""
+    sd = scenes.random_instances()
+    gs = solhip.GpuScene(ctx, sd)
+    W, H = 80, 48
+    acc = ctx.upload(np.zeros((H, W, 4), np.float32))
+    img = solhip.Images(acc.h, 0, 0, 0, 0, 0)
+    spec = _spec(solhip.PROG_PATHTRACE, sky=2, spp=4, depth=5, nee=1, rr=2)
+    spec.sample_streams = 1
+    cache = str(tmp_path / "tune.txt")
+    print("autotune with sample streams", flush=True)
+    best, ms = ctx.autotune_pathtrace(gs.tlas, gs.tables, sd.uniforms(W, H, 0), img, W, H, spec, 1, cache)
+    assert best.wf_slices in (0, 1, 2, 4, 8) and best.sample_streams == 1 and ms > 0
+    lines = open(cache).read().splitlines()
+    assert len(lines) == 1 and "streams=1" in lines[0] and "slices=" in lines[0]
+    again, _ = ctx.autotune_pathtrace(gs.tlas, gs.tables, sd.uniforms(W, H, 0), img, W, H, spec, 1, cache)
+    assert again.wf_slices == best.wf_slices and len(open(cache).read().splitlines()) == 1
+    a0, _, _ = _render(ctx, gs, sd, solhip.PROG_PATHTRACE, spec, W, H, 1)
+    a1, _, _ = _render(ctx, gs, sd, solhip.PROG_PATHTRACE, best, W, H, 1)
+    assert (_bits(a0) != _bits(a1)).sum() == 0
