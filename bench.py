@@ -110,7 +110,7 @@ def main():
         copy_calibration(a.copy_calibration)
         return
     if a.gpus != 1:
-        print("bench.py measures one GPU only (this build has no multi-GPU path); got --gpus %d" % a.gpus,
+        print("bench.py measures one GPU only (the library has no multi-GPU path); got --gpus %d" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
     if a.workload.startswith("pathtrace"):

@@ -795,7 +795,7 @@ void sr_render_ao_glsl(const sr_scene *s, const sr_tables *tb, const sr_uniforms
 }
 
 /* ------------------------------------------------------------------------ */
-/* plain ray-traced AO (the north-star AO workload, built on ao.rgen's      */
+/* plain ray-traced AO (the 2 M-triangle AO workload, built on ao.rgen's   */
 /* camera/seed and ao.rchit's surface reconstruction):                      */
 /*   rng = tea(pixel, frame); jitter = 2 rands; primary ray with            */
 /*   preparePayload's range; on a hit, n_ao cosine directions (2 rands      */
