@@ -5,11 +5,12 @@
 //
 //   pathtrace --model scene.gltf [--sky] [--size 1280x720] [--frames 16]
 //             [--out image.ppm] [--accum accum.f32] [--dump dir] [--device 0]
-//             [--wavefront [--nee] [--rr N] [--spp N] [--depth N]]
+//             [--wavefront [--nee] [--rr N] [--spp N] [--depth N] [--streams]]
 //
 // --wavefront runs the wavefront path tracer (same image as the default
-// program without --nee / --rr); --nee adds next-event estimation, --rr N
-// Russian roulette from bounce N.
+// program without --nee / --rr / --streams); --nee adds next-event
+// estimation, --rr N Russian roulette from bounce N, --streams a random
+// stream per sample (many samples of every pixel traced at once).
 #include <chrono>
 
 #include "common.hpp"
@@ -42,6 +43,7 @@ int main(int argc, char **argv)
         spec.rr_start = (uint32_t)std::atoi(example::arg(argc, argv, "--rr", "0").c_str());
         spec.spp_per_frame = (uint32_t)std::atoi(example::arg(argc, argv, "--spp", "8").c_str());
         spec.max_depth = (uint32_t)std::atoi(example::arg(argc, argv, "--depth", "32").c_str());
+        spec.sample_streams = example::has_flag(argc, argv, "--streams") ? 1u : 0u;
         ray::Pipeline pipeline(ctx,
                                ray::PipelineInfo()
                                    .shader(wavefront ? "glsl/pathtrace_wavefront.rgen" : "glsl/pathtrace.rgen",

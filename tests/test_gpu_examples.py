@@ -100,20 +100,21 @@ def test_ray_ao_example_matches_oracle(oracle, tmp_path):
     assert 0.0 < ref[..., 0].max() <= 1.0
 
 
-def test_pathtrace_example_wavefront_nee_matches_oracle(oracle, tmp_path):
+@pytest.mark.parametrize("streams", [0, 1])
+def test_pathtrace_example_wavefront_nee_matches_oracle(oracle, tmp_path, streams):
     W, H, F = 48, 40, 2
     d = str(tmp_path / "dump")
     acc_file = str(tmp_path / "accum.f32")
     res = _run("pathtrace", ["--model", CORNELL, "--size", "%dx%d" % (W, H), "--frames", str(F), "--dump", d,
                              "--accum", acc_file, "--out", str(tmp_path / "pt.ppm"), "--wavefront", "--nee",
-                             "--rr", "2", "--spp", "2", "--depth", "4"])
+                             "--rr", "2", "--spp", "3", "--depth", "4"] + (["--streams"] if streams else []))
     assert res["frames"] == F and res["overflow"] == 0
     sd = _scene_from_dump(d)
     sc = oracle.Scene(sd)
     ref = np.zeros((H, W, 4), np.float32)
     rays = 0
     for f in range(F):
-        rays += sc.render_pathtrace(_uniforms(d, f), oracle.PTConfig(0, 2, 4, 2, 1), 0, ref, None, W, H)
+        rays += sc.render_pathtrace(_uniforms(d, f), oracle.PTConfig(0, 3, 4, 2, 1, streams), 0, ref, None, W, H)
     gpu = np.fromfile(acc_file, np.float32).reshape(H, W, 4)
     assert np.count_nonzero(gpu.view(np.uint32) != ref.view(np.uint32)) == 0
     assert res["rays"] == rays
