@@ -941,15 +941,14 @@ int32_t solhip_trace(solhip_ctx ctx, solhip_stream st, solhip_program ph, solhip
     }
     switch (p->kind_id) {
     case SOLHIP_PROG_AO: {
-        const uint64_t rb = ao_ray_bytes(npix, a.n_ao), hb = ao_head_bytes();
+        const uint64_t hb = ao_head_bytes();
         if (p->queue.reserve(npix * sizeof(AoEntry)) || p->qcount.reserve(256) || p->occl.reserve(npix * a.n_ao) ||
-            (rb && p->aorays.reserve(rb)) || (hb && p->aoheads.reserve(hb)))
+            (hb && p->aoheads.reserve(hb)))
             return SOLHIP_ERROR_OUT_OF_MEMORY;
         a.aoheads = hb ? (uint32_t *)p->aoheads.ptr : nullptr;
         a.queue = (AoEntry *)p->queue.ptr;
         a.qcount = (uint32_t *)p->qcount.ptr;
         a.occl = (uint8_t *)p->occl.ptr;
-        a.aorays = rb ? (QRay *)p->aorays.ptr : nullptr;
         a.ncu = c->ncu;
         int rc = launch_fill(s, a.qcount, 2, 0u);
         if (rc) return rc;

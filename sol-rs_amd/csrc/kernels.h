@@ -37,8 +37,6 @@ struct Timer {
     virtual ~Timer() = default;
 };
 
-struct QRay;
-
 struct ProgArgs {
     SceneView sv;
     DevTables tb;
@@ -57,7 +55,6 @@ struct ProgArgs {
     AoEntry *queue;
     uint32_t *qcount;       // [0] queue entries, [1] ray cursor of the persistent AO kernel
     uint8_t *occl;          // per AO ray: occluded (persistent AO kernel)
-    QRay *aorays;           // per AO ray: origin, pixel, direction (written by k_ao_primary; nullptr: none)
     uint32_t *aoheads;      // fetch heads of the persistent AO kernel's partitions, 128 B apart
     uint32_t ncu;
     // pathtrace
@@ -156,8 +153,6 @@ struct WfArgs {
 int launch_trace_rays(hipStream_t s, const SceneView &sv, const solhip_ray *rays, solhip_hit *hits, uint32_t n,
                       uint32_t flags, Counters *counters, Timer *timer);
 int launch_ao(hipStream_t s, const ProgArgs &a, Timer *timer);
-// bytes of the AO ray buffer the AO program needs for npix pixels (0: none)
-uint64_t ao_ray_bytes(uint64_t npix, uint32_t n_ao);
 // bytes of the persistent AO kernel's fetch heads
 uint64_t ao_head_bytes();
 int launch_ao_glsl(hipStream_t s, const ProgArgs &a, Timer *timer);

@@ -112,7 +112,7 @@ struct Tlas : Obj {
 struct Program : Obj {
     uint32_t kind_id = 0;
     solhip_spec spec{};
-    DevBuf queue, qcount, occl, aorays, aoheads;
+    DevBuf queue, qcount, occl, aoheads;
     // SOLHIP_PROG_PATHTRACE: path state, queues, emitter table, frame constants
     DevBuf st0, st1, psum, q0, q1, hits, sq, scontrib, counts, heads, lights, light_offsets, fc;
     // radiance cache (spec.nrc)
@@ -151,7 +151,7 @@ struct Program : Obj {
         }
         if (fork_ev) (void)hipEventDestroy(fork_ev);
         fork_ev = nullptr;
-        for (DevBuf *b : {&queue, &qcount, &occl, &aorays, &aoheads, &st0, &st1, &psum, &q0, &q1, &hits, &sq, &scontrib, &counts, &heads,
+        for (DevBuf *b : {&queue, &qcount, &occl, &aoheads, &st0, &st1, &psum, &q0, &q1, &hits, &sq, &scontrib, &counts, &heads,
                           &lights, &light_offsets, &fc, &nq_feat, &nq_rec, &n_counts, &n_tidx, &n_tpath, &n_trec_n,
                           &n_tfeat, &n_trec, &n_bfeat, &n_btarget})
             b->release();

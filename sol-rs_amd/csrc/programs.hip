@@ -32,11 +32,6 @@ namespace solhip {
 #ifndef SOLHIP_AO_REFILL
 #define SOLHIP_AO_REFILL 16
 #endif
-// 1: k_ao_primary writes every AO ray (32 B) and the persistent kernel's
-// refill only loads it; 0: the refill samples the direction itself
-#ifndef SOLHIP_AO_RAYBUF
-#define SOLHIP_AO_RAYBUF 0
-#endif
 // fetch partitions of the persistent AO kernel (fetch.h)
 #ifndef SOLHIP_AO_PARTS
 #define SOLHIP_AO_PARTS 64
@@ -204,19 +199,13 @@ __global__ void __launch_bounds__(kProgBlock) SOLHIP_AO_RAYS_OCCUPANCY k_ao_rays
             if (!has) {
                 const uint32_t my = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
                 if (my < end) {
-                    if (A.aorays) {
-                        const QRay r = A.aorays[my];
-                        trav_init(st, A.sv, v3(r.ox, r.oy, r.oz), v3(r.dx, r.dy, r.dz), A.ao_tmin, A.ao_tmax);
-                        pix = r.path;
-                    } else {
-                        const AoEntry e = A.queue[my >> shift];
-                        uint32_t rng = pcg_advance(e.rng, 2u * (my & (A.n_ao - 1u)));
-                        const float xi0 = next_rand(rng);
-                        const float xi1 = next_rand(rng);
-                        const V3 dir = sample_cosine(v3(e.nx, e.ny, e.nz), xi0, xi1);
-                        trav_init(st, A.sv, v3(e.px, e.py, e.pz), dir, A.ao_tmin, A.ao_tmax);
-                        pix = e.pixel;
-                    }
+                    const AoEntry e = A.queue[my >> shift];
+                    uint32_t rng = pcg_advance(e.rng, 2u * (my & (A.n_ao - 1u)));
+                    const float xi0 = next_rand(rng);
+                    const float xi1 = next_rand(rng);
+                    const V3 dir = sample_cosine(v3(e.nx, e.ny, e.nz), xi0, xi1);
+                    trav_init(st, A.sv, v3(e.px, e.py, e.pz), dir, A.ao_tmin, A.ao_tmax);
+                    pix = e.pixel;
                     rid = my;
                     has = true;
                 }
@@ -272,12 +261,6 @@ static int launch_status(const char *what)
     } while (0)
 
 uint64_t ao_head_bytes() { return SOLHIP_AO_REFILL ? (uint64_t)SOLHIP_AO_PARTS * kHeadStride * 4 : 0; }
-
-uint64_t ao_ray_bytes(uint64_t npix, uint32_t n_ao)
-{
-    constexpr bool kRayBuf = SOLHIP_AO_RAYBUF != 0 && SOLHIP_AO_REFILL != 0;
-    return kRayBuf ? npix * n_ao * sizeof(QRay) : 0;
-}
 
 int launch_ao(hipStream_t s, const ProgArgs &a, Timer *timer)
 {
