@@ -68,7 +68,8 @@ def parse_args():
     ap.add_argument("--jobs", type=int, default=40, help="server-c3: requests")
     ap.add_argument("--rate", type=float, default=200.0, help="server-c3: requests per second")
     ap.add_argument("--streams", type=int, default=1, choices=(0, 1),
-                    help="pathtrace-c0/c2: 1 = a random stream per sample, 0 = one per pixel (pathtrace.rgen)")
+                    help="pathtrace-c0/c2, server-c3: 1 = a random stream per sample, 0 = one per pixel "
+                         "(pathtrace.rgen)")
     ap.add_argument("--per-pass", type=int, default=0, help="pathtrace-c0/c2: samples per pass (0 = library default)")
     ap.add_argument("--slices", type=int, default=0, help="pathtrace-c0/c2: image slices (0 = library default)")
     ap.add_argument("--any-hit", action="store_true",
@@ -326,16 +327,17 @@ def bench_server(a):
     gss = [solhip.GpuScene(ctx, sd) for sd in sds]
     ctx.sync()
     warm = serve.draw_requests(4, 1000.0, seed=1, resolutions=[(256, 256)], spp_max=4)
-    serve.run_load(ctx, gss, sds, warm)
+    serve.run_load(ctx, gss, sds, warm, streams=a.streams)
     reqs = serve.draw_requests(a.jobs, a.rate, seed=3)
-    stats = serve.run_load(ctx, gss, sds, reqs)
+    stats = serve.run_load(ctx, gss, sds, reqs, streams=a.streams)
     result = {
         "metric": "render-request latency on 1 GPU", "value": round(stats["latency_ms_p50"], 2), "unit": "ms (p50)",
         "n_gpus": 1, "steps": a.jobs, "warmup": 4, "ms_per_step": round(stats["makespan_s"] * 1e3 / a.jobs, 3),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "render server, Poisson %.0f req/s, %d requests over Cornell / 2.1M-triangle "
                                "heightfield / 10k instances, resolutions 256^2..1080p, spp log-uniform 1-256, "
-                               "depth 4-8, NEE + roulette" % (a.rate, a.jobs), "parallelism": "single GPU"},
+                               "depth 4-8, NEE + roulette" % (a.rate, a.jobs), "sample_streams": a.streams,
+                   "parallelism": "single GPU"},
         "extra": stats,
     }
     ctx.close()

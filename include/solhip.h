@@ -333,7 +333,11 @@ int32_t solhip_pathtrace_autotune(solhip_ctx ctx, solhip_stream s, solhip_tlas t
 typedef struct {
     uint32_t scene;                  /* from solhip_server_scene */
     uint32_t W, H;
-    solhip_spec spec;                /* SOLHIP_PROG_PATHTRACE settings (launch-shape fields ignored) */
+    solhip_spec spec;                /* SOLHIP_PROG_PATHTRACE settings (launch-shape fields ignored).
+                                        With sample_streams, wf_samples_per_pass is the job's slots
+                                        per pixel, each running every lanes-th sample (0 = about 16
+                                        samples per slot within ~4 M slots); the image equals
+                                        solhip_trace's for any value */
     solhip_scene_uniforms uniforms;
     int32_t accum_start_frame;
     uint32_t reserved;

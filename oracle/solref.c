@@ -1077,6 +1077,8 @@ static void pt_cfg_pixel(const sr_scene *s, const sr_tables *tb, const sr_unifor
     const int sun = p->enable_sky != 2;
     const int nee = p->nee && nl > 0 && total > 0.0f;
     float pixelColor[3] = {0.0f, 0.0f, 0.0f};
+    int64_t fix[3] = {0, 0, 0};
+    int fix_bad = 0;
     uint32_t rng = sr_tea(x + y * W, un->frame[2]);
     for (int i = 0; i < sampleCount; i++) {
         /* pathtrace.rgen:47 seeds once per pixel; with sample_streams every
@@ -1189,14 +1191,25 @@ static void pt_cfg_pixel(const sr_scene *s, const sr_tables *tb, const sr_unifor
                 for (int a = 0; a < 3; ++a) T[a] = T[a] / q;
             }
         }
-        for (int a = 0; a < 3; ++a) pixelColor[a] = pixelColor[a] + L[a];
+        if (p->sample_streams) {
+            /* order-free sum: each sample in fixed point, 2^20 units per 1.0,
+             * rounded to nearest even; 2^19 or more (or NaN / Inf) is bad */
+            for (int a = 0; a < 3; ++a) {
+                if (!(fabsf(L[a]) < 0x1p19f)) fix_bad = 1;
+                else fix[a] += (int64_t)rintf(L[a] * 0x1p20f);
+            }
+        } else {
+            for (int a = 0; a < 3; ++a) pixelColor[a] = pixelColor[a] + L[a];
+        }
     }
+    if (p->sample_streams)
+        for (int a = 0; a < 3; ++a) pixelColor[a] = (float)((double)fix[a] * 0x1p-20);
     float inv = 1.0f / (float)sampleCount;
     for (int a = 0; a < 3; ++a) pixelColor[a] = inv * pixelColor[a];
     float alpha = 1.0f / (float)(un->frame[2] + 1u - (uint32_t)accum_start);
     float *px = &accum[4 * ((size_t)y * W + x)];
     float mixed[3];
-    int bad = 0;
+    int bad = fix_bad;
     for (int a = 0; a < 3; ++a) {
         mixed[a] = mixf(px[a], pixelColor[a], alpha);
         if (isnan(mixed[a]) || isinf(mixed[a])) bad = 1;

@@ -186,7 +186,8 @@ typedef struct {
     uint32_t max_bounces;
     uint32_t rr_start;    /* 0 = no Russian roulette */
     uint32_t nee;         /* 1 = next-event estimation */
-    uint32_t sample_streams; /* 1 = sample i starts its own stream tea(x + y*W + i*W*H, frame) */
+    uint32_t sample_streams; /* 1 = sample i starts its own stream tea(x + y*W + i*W*H, frame), and
+                                the samples are summed in fixed point (2^20 units per 1.0, int64) */
 } sr_pt_config;
 void sr_render_pathtrace(const sr_scene *s, const sr_tables *tb, const sr_uniforms *u,
                          const sr_pt_config *p, int32_t accum_start_frame,

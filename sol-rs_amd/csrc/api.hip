@@ -17,6 +17,7 @@
 #include "common.h"
 #include "objects.h"
 #include "kernels.h"
+#include "accum.h"
 #include "traverse.h"
 
 namespace solhip {
@@ -300,6 +301,7 @@ int trace_wavefront(Context *c, hipStream_t s, Program *p, Tlas *t, const ProgAr
     if (p->spec.nrc || c->timer_or_null()) nslices = 1;
     nslices = std::max(1u, std::min(nslices, tiles32));
     if (p->st0.reserve(np * 16) || p->st1.reserve(np * 16) || p->psum.reserve(npix * 16) ||
+        (w.streams && p->fsum.reserve(npix * 32)) ||
         p->q0.reserve(ns * sizeof(QRay)) || p->q1.reserve(ns * sizeof(QRay)) || p->hits.reserve(ns * sizeof(solhip_hit)) ||
         p->sq.reserve(ns * sizeof(QRay)) || p->scontrib.reserve(ns * 16) || p->counts.reserve(256ull * nslices) ||
         p->heads.reserve(nheads * wf_head_words() * 4 * nslices) || p->fc.reserve(sizeof(FrameConsts)))
@@ -313,6 +315,7 @@ int trace_wavefront(Context *c, hipStream_t s, Program *p, Tlas *t, const ProgAr
     w.st0 = (float4 *)p->st0.ptr;
     w.st1 = (float4 *)p->st1.ptr;
     w.psum = (float4 *)p->psum.ptr;
+    w.fsum = w.streams ? (FixSum *)p->fsum.ptr : nullptr;
     w.q[0] = (QRay *)p->q0.ptr;
     w.q[1] = (QRay *)p->q1.ptr;
     w.hits = (solhip_hit *)p->hits.ptr;

@@ -132,8 +132,10 @@ struct WfArgs {
     uint32_t nslots;                  // slots of the first queue (8x8 pixel tiles * 64)
     uint32_t slot_base;               // first slot of this image slice (0: the whole image)
     // path state, S * npix each: (T, rng), (L, depth | count_emission << 16);
-    // per-pixel sum of the finished samples
+    // per-pixel sum of the finished samples: float (one stream per pixel) or
+    // order-free fixed point (per-sample streams, accum.h)
     float4 *st0, *st1, *psum;
+    struct FixSum *fsum;
     QRay *q[2];                       // extension queues (ping-pong by bounce)
     solhip_hit *hits;                 // closest hits, by slot of the current queue
     QRay *sq;                         // shadow queue

@@ -114,7 +114,7 @@ struct Program : Obj {
     solhip_spec spec{};
     DevBuf queue, qcount, occl, aoheads;
     // SOLHIP_PROG_PATHTRACE: path state, queues, emitter table, frame constants
-    DevBuf st0, st1, psum, q0, q1, hits, sq, scontrib, counts, heads, lights, light_offsets, fc;
+    DevBuf st0, st1, psum, fsum, q0, q1, hits, sq, scontrib, counts, heads, lights, light_offsets, fc;
     // radiance cache (spec.nrc)
     Mlp *nrc_mlp = nullptr;
     DevBuf nq_feat, nq_rec, n_counts, n_tidx, n_tpath, n_trec_n, n_tfeat, n_trec, n_bfeat, n_btarget;
@@ -151,7 +151,7 @@ struct Program : Obj {
         }
         if (fork_ev) (void)hipEventDestroy(fork_ev);
         fork_ev = nullptr;
-        for (DevBuf *b : {&queue, &qcount, &occl, &aoheads, &st0, &st1, &psum, &q0, &q1, &hits, &sq, &scontrib, &counts, &heads,
+        for (DevBuf *b : {&queue, &qcount, &occl, &aoheads, &st0, &st1, &psum, &fsum, &q0, &q1, &hits, &sq, &scontrib, &counts, &heads,
                           &lights, &light_offsets, &fc, &nq_feat, &nq_rec, &n_counts, &n_tidx, &n_tpath, &n_trec_n,
                           &n_tfeat, &n_trec, &n_bfeat, &n_btarget})
             b->release();

@@ -32,6 +32,7 @@
 #include "traverse.h"
 #include "traverse_step.h"
 #include "fetch.h"
+#include "accum.h"
 
 namespace solhip {
 
@@ -62,11 +63,14 @@ struct JobDev {
     float alpha, rmax;
     uint32_t base;      // first slot
     uint32_t streams;   // spec.sample_streams: sample s starts tea(p + s * W * H, frame)
+    uint32_t lanes;     // slots per pixel (per-sample streams): slot base + j * npix + p runs samples j, j + lanes, ...
+    uint32_t npix;      // W * H
 };
 
 struct SrvArgs {
     const JobDev *jobs;
     float4 *st0, *st1, *psum;
+    FixSum *fsum;         // per-sample streams: per-slot order-free sums (lane 0's count: lanes done)
     uint32_t *pmeta;      // job | sample << 8
     QRay *q[2];
     solhip_hit *hits;
@@ -131,23 +135,33 @@ __global__ void k_srv_set_job(JobDev *jobs, uint32_t *done, uint32_t j, JobDev v
 }
 
 // first sample of every pixel of job j, appended to queue cur (8x8 tiles)
+// the first sample of every slot of job j (lane-major, 8x8 tiles within a
+// lane), appended to queue cur; nslots = lanes * tile slots
 __global__ void k_srv_admit(SrvArgs A, uint32_t j, uint32_t cur, uint32_t nslots)
 {
     const JobDev &J = A.jobs[j];
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t tiles_x = (J.W + 7) >> 3, tile = t >> 6, lane = t & 63u;
+    const uint32_t tslots = nslots / J.lanes, ln = t / tslots, tt = t - ln * tslots;
+    const uint32_t tiles_x = (J.W + 7) >> 3, tile = tt >> 6, lane = tt & 63u;
     const uint32_t x = (tile % tiles_x) * 8 + (lane & 7u), y = (tile / tiles_x) * 8 + (lane >> 3);
     const bool valid = t < nslots && x < J.W && y < J.H;
     QRay r;
     if (valid) {
-        const uint32_t slot = J.base + y * J.W + x;
-        uint32_t rng = tea(x + y * J.W, J.frame);   // pathtrace.rgen:47
+        const uint32_t p = y * J.W + x, slot = J.base + ln * J.npix + p;
+        // pathtrace.rgen:47; lane ln starts sample ln of its own stream
+        uint32_t rng = tea(p + ln * J.npix, J.frame);
         float4 s0, s1;
         r = start_sample(J, slot, x, y, rng, s0, s1);
         A.st0[slot] = s0;
         A.st1[slot] = s1;
-        A.psum[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        A.pmeta[slot] = j;
+        if (J.streams) {
+            FixSum fs;
+            fix_clear(fs);
+            A.fsum[slot] = fs;
+        } else {
+            A.psum[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        A.pmeta[slot] = j | (ln << 8);
     }
     const uint32_t at = srv_wave_append(A.counts + cur, valid);
     if (valid) A.q[cur][at] = r;
@@ -288,28 +302,71 @@ __global__ void __launch_bounds__(kSrvBlock) k_srv_retire(SrvArgs A, uint32_t cu
         if (i < cnt) {
             const uint32_t slot = A.retire[i];
             const uint32_t meta = A.pmeta[slot], job = meta & 0xFFu;
-            const uint32_t sample = (meta >> 8) + 1u;
             const JobDev &J = A.jobs[job];
-            const uint32_t p = slot - J.base, x = p % J.W, y = p / J.W;
-            const float4 s1 = A.st1[slot], ps = A.psum[slot];
-            const float pc0 = ps.x + s1.x, pc1 = ps.y + s1.y, pc2 = ps.z + s1.z;
+            const uint32_t sample = (meta >> 8) + J.lanes;   // this slot's next sample
+            const uint32_t k = slot - J.base, ln = k / J.npix, p = k - ln * J.npix, x = p % J.W, y = p / J.W;
+            const float4 s1 = A.st1[slot];
+            float pc0 = 0.0f, pc1 = 0.0f, pc2 = 0.0f;
+            FixSum fs;
+            fix_clear(fs);
+            bool last = true;
+            if (J.streams) {
+                fs = A.fsum[slot];
+                fix_add(fs, s1.x, s1.y, s1.z);
+            } else {
+                const float4 ps = A.psum[slot];
+                pc0 = ps.x + s1.x;
+                pc1 = ps.y + s1.y;
+                pc2 = ps.z + s1.z;
+            }
             if (sample < J.spp) {
-                A.psum[slot] = make_float4(pc0, pc1, pc2, 0.0f);
-                uint32_t rng = J.streams ? tea(p + sample * (J.W * J.H), J.frame) : as_u(A.st0[slot].w);
+                if (J.streams) {
+                    A.fsum[slot].c[0] = fs.c[0];
+                    A.fsum[slot].c[1] = fs.c[1];
+                    A.fsum[slot].c[2] = fs.c[2];
+                    A.fsum[slot].bad = fs.bad;
+                } else {
+                    A.psum[slot] = make_float4(pc0, pc1, pc2, 0.0f);
+                }
+                uint32_t rng = J.streams ? tea(p + sample * J.npix, J.frame) : as_u(A.st0[slot].w);
                 float4 s0n, s1n;
                 r = start_sample(J, slot, x, y, rng, s0n, s1n);
                 A.st0[slot] = s0n;
                 A.st1[slot] = s1n;
                 A.pmeta[slot] = job | (sample << 8);
                 again = true;
-            } else {
+                last = false;
+            } else if (J.streams) {
+                // this lane of the pixel is done: publish its sum; the pixel's
+                // last lane (counted on lane 0's record) adds all of them up.
+                // The count is only ever changed atomically, so the partial
+                // sum is stored field by field.
+                A.fsum[slot].c[0] = fs.c[0];
+                A.fsum[slot].c[1] = fs.c[1];
+                A.fsum[slot].c[2] = fs.c[2];
+                A.fsum[slot].bad = fs.bad;
+                if (J.lanes > 1) {
+                    __threadfence();
+                    last = atomicAdd(&A.fsum[J.base + p].count, 1u) == J.lanes - 1u;
+                    if (last) {
+                        __threadfence();
+                        for (uint32_t l = 0; l < J.lanes; ++l)
+                            if (l != ln) fix_merge(fs, A.fsum[J.base + l * J.npix + p]);
+                    }
+                }
+                pc0 = fix_value(fs.c[0]);
+                pc1 = fix_value(fs.c[1]);
+                pc2 = fix_value(fs.c[2]);
+            }
+            if (!again && last) {
                 // pathtrace.rgen:88-102
                 const float inv = 1.0f / (float)J.spp;
                 const float c0 = inv * pc0, c1 = inv * pc1, c2 = inv * pc2;
                 float4 *px = J.accum + p;
                 const float4 old = *px;
                 float m0 = mixf(old.x, c0, J.alpha), m1 = mixf(old.y, c1, J.alpha), m2 = mixf(old.z, c2, J.alpha);
-                const bool bad = isnan(m0) || isnan(m1) || isnan(m2) || isinf(m0) || isinf(m1) || isinf(m2);
+                const bool bad = (J.streams && fs.bad) || isnan(m0) || isnan(m1) || isnan(m2) || isinf(m0) ||
+                                 isinf(m1) || isinf(m2);
                 if (bad) {
                     m0 = old.x;
                     m1 = old.y;
@@ -347,9 +404,15 @@ double now_ms(std::chrono::steady_clock::time_point t0)
 
 }  // namespace
 
-// bytes of device state per path slot: st0, st1, psum (48), meta (4), two
-// extension queues (64), hits (20), shadow queue + contribution (48), retire (4)
-constexpr uint64_t kSlotBytes = 48 + 4 + 64 + 20 + 48 + 4;
+// bytes of device state per path slot: st0, st1, psum (48), order-free sum
+// (32), meta (4), two extension queues (64), hits (20), shadow queue +
+// contribution (48), retire (4)
+constexpr uint64_t kSlotBytes = 48 + 32 + 4 + 64 + 20 + 48 + 4;
+// slots per pixel of a job with per-sample streams when its spec leaves it
+// open: enough lanes that a lane runs about kLaneSamples samples, within
+// about kJobSlots slots for the job
+constexpr uint32_t kLaneSamples = 16;
+constexpr uint64_t kJobSlots = 4ull << 20;
 
 struct SrvScene {
     SceneView sv;
@@ -364,13 +427,14 @@ struct SrvJob {
     uint32_t state = SOLHIP_JOB_QUEUED;
     double t_submit = 0, t_admit = 0, t_done = 0;
     uint32_t base = 0, npix = 0, jidx = 0;
+    uint32_t lanes = 1, slots = 0;   // slots per pixel, slots of the job (lanes * npix)
 };
 
 struct Server : Obj {
     Context *cx = nullptr;
     hipStream_t stream = nullptr;
     uint32_t max_slots = 0, max_jobs = 0, max_stack = 1;
-    DevBuf jobs, st0, st1, psum, pmeta, q0, q1, hits, sq, scontrib, retire, counts, heads, done;
+    DevBuf jobs, st0, st1, psum, fsum, pmeta, q0, q1, hits, sq, scontrib, retire, counts, heads, done;
     uint32_t *done_host = nullptr;   // pinned copy of the done counters
     std::vector<SrvScene *> scenes;
     std::map<uint64_t, SrvJob> all;
@@ -389,6 +453,7 @@ struct Server : Obj {
         a.st0 = (float4 *)st0.ptr;
         a.st1 = (float4 *)st1.ptr;
         a.psum = (float4 *)psum.ptr;
+        a.fsum = (FixSum *)fsum.ptr;
         a.pmeta = (uint32_t *)pmeta.ptr;
         a.q[0] = (QRay *)q0.ptr;
         a.q[1] = (QRay *)q1.ptr;
@@ -432,7 +497,7 @@ struct Server : Obj {
     }
     void release_all()
     {
-        for (DevBuf *b : {&jobs, &st0, &st1, &psum, &pmeta, &q0, &q1, &hits, &sq, &scontrib, &retire, &counts, &heads,
+        for (DevBuf *b : {&jobs, &st0, &st1, &psum, &fsum, &pmeta, &q0, &q1, &hits, &sq, &scontrib, &retire, &counts, &heads,
                           &done})
             b->release();
         for (SrvScene *sc : scenes) {
@@ -457,12 +522,12 @@ struct Server : Obj {
             while (jidx < max_jobs && running_by_idx[jidx]) ++jidx;
             if (jidx == max_jobs) return 0;
             uint32_t base = 0;
-            if (!alloc_range(jb.npix, base)) return 0;
+            if (!alloc_range(jb.slots, base)) return 0;
             const SrvScene *sc = scenes[jb.desc.scene];
             Buffer *acc = get<Buffer>(jb.desc.accum, Kind::Buffer);
             Buffer *out = jb.desc.out_rgba8 ? get<Buffer>(jb.desc.out_rgba8, Kind::Buffer) : nullptr;
             if (!acc) {
-                free_range(base, jb.npix);
+                free_range(base, jb.slots);
                 return bad_handle("job accum buffer");
             }
             JobDev v;
@@ -489,9 +554,11 @@ struct Server : Obj {
             v.alpha = 1.0f / (float)denom;   // pathtrace.rgen:90
             v.rmax = 10000.0f;
             v.base = base;
+            v.lanes = jb.lanes;
+            v.npix = jb.npix;
             hipLaunchKernelGGL(k_srv_set_job, dim3(1), dim3(64), 0, stream, (JobDev *)jobs.ptr, (uint32_t *)done.ptr,
                                jidx, v);
-            const uint32_t tiles = ((v.W + 7) / 8) * ((v.H + 7) / 8), nslots = tiles * 64;
+            const uint32_t tiles = ((v.W + 7) / 8) * ((v.H + 7) / 8), nslots = tiles * 64 * jb.lanes;
             Timer *timer = cx->timer_or_null();
             if (timer) timer->begin(stream, SOLHIP_KERNEL_SRV_ADMIT);
             hipLaunchKernelGGL(k_srv_admit, dim3((nslots + kSrvBlock - 1) / kSrvBlock), dim3(kSrvBlock), 0, stream,
@@ -547,7 +614,7 @@ struct Server : Obj {
             if (done_host[j] >= jb.npix) {
                 jb.state = SOLHIP_JOB_DONE;
                 jb.t_done = t;
-                free_range(jb.base, jb.npix);
+                free_range(jb.base, jb.slots);
                 running_by_idx[j] = 0;
             }
         }
@@ -586,7 +653,8 @@ int32_t solhip_server_create(solhip_ctx ctx, uint64_t hbm_budget, uint32_t max_j
     sv->t0 = std::chrono::steady_clock::now();
     const uint64_t n = slots;
     bool fail = sv->jobs.reserve((uint64_t)max_jobs * sizeof(JobDev)) || sv->st0.reserve(n * 16) ||
-                sv->st1.reserve(n * 16) || sv->psum.reserve(n * 16) || sv->pmeta.reserve(n * 4) ||
+                sv->st1.reserve(n * 16) || sv->psum.reserve(n * 16) || sv->fsum.reserve(n * sizeof(FixSum)) ||
+                sv->pmeta.reserve(n * 4) ||
                 sv->q0.reserve(n * sizeof(QRay)) || sv->q1.reserve(n * sizeof(QRay)) ||
                 sv->hits.reserve(n * sizeof(solhip_hit)) || sv->sq.reserve(n * sizeof(QRay)) ||
                 sv->scontrib.reserve(n * 16) || sv->retire.reserve(n * 4) || sv->counts.reserve(256) ||
@@ -692,6 +760,15 @@ int32_t solhip_server_submit(solhip_ctx ctx, solhip_server h, const solhip_job_d
     SrvJob jb;
     jb.desc = *job;
     jb.npix = (uint32_t)npix;
+    if (job->spec.sample_streams) {
+        const uint32_t spp = job->spec.spp_per_frame;
+        uint64_t lanes = job->spec.wf_samples_per_pass;
+        if (lanes == 0)
+            lanes = std::min<uint64_t>((spp + kLaneSamples - 1) / kLaneSamples, std::max<uint64_t>(1, kJobSlots / npix));
+        lanes = std::max<uint64_t>(1, std::min<uint64_t>({lanes, spp, sv->max_slots / npix}));
+        jb.lanes = (uint32_t)lanes;
+    }
+    jb.slots = jb.lanes * jb.npix;
     jb.t_submit = now_ms(sv->t0);
     const uint64_t id = sv->next_id++;
     sv->all[id] = jb;

@@ -31,6 +31,7 @@
 #include "traverse.h"
 #include "traverse_step.h"
 #include "fetch.h"
+#include "accum.h"
 
 namespace solhip {
 
@@ -112,7 +113,7 @@ __global__ void k_pt_generate(WfArgs A, uint32_t pass)
         // one sample per pass: the RNG runs on from the previous sample
         rng = as_u(A.st0[p].w);
     }
-    if (A.S == 1 && pass > 0) {
+    if (!A.streams && pass > 0) {
         // the previous sample's radiance joins the sum
         const float4 s1 = A.st1[p];
         const float4 ps = pass == 1 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : A.psum[pix];
@@ -135,22 +136,29 @@ __global__ void k_pt_generate(WfArgs A, uint32_t pass)
     A.q[0][e] = r;
 }
 
-// the radiance of pass `pass`'s samples joins each pixel's sum, in sample
-// order (the order of the one-sample-per-pass sums); S > 1 only, before the
-// next pass's generate overwrites the path state
+// per-sample streams: the radiance of pass `pass`'s samples joins each
+// pixel's order-free sum (accum.h), before the next pass's generate
+// overwrites the path state
+__device__ __forceinline__ void fold_pass(const WfArgs &A, uint32_t pass, uint32_t pix, FixSum &fs)
+{
+    const uint32_t sk = pass_samples(A, pass);
+    for (uint32_t sl = 0; sl < sk; ++sl) {
+        const float4 s1 = A.st1[sl * A.npix + pix];
+        fix_add(fs, s1.x, s1.y, s1.z);
+    }
+}
+
 __global__ void k_pt_fold(WfArgs A, uint32_t pass)
 {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t x, y;
     if (slot >= A.nslots || !slot_pixel(A.slot_base + slot, A.W, A.H, x, y)) return;
     const uint32_t pix = y * A.W + x;
-    float4 ps = pass == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : A.psum[pix];
-    const uint32_t sk = pass_samples(A, pass);
-    for (uint32_t sl = 0; sl < sk; ++sl) {
-        const float4 s1 = A.st1[sl * A.npix + pix];
-        ps = make_float4(ps.x + s1.x, ps.y + s1.y, ps.z + s1.z, 0.0f);
-    }
-    A.psum[pix] = ps;
+    FixSum fs;
+    if (pass == 0) fix_clear(fs);
+    else fs = A.fsum[pix];
+    fold_pass(A, pass, pix, fs);
+    A.fsum[pix] = fs;
 }
 
 // register budget of the traversal kernels: waves per SIMD they must fit
@@ -518,14 +526,27 @@ __global__ void k_pt_finalize(WfArgs A)
     uint32_t x, y;
     if (slot >= A.nslots || !slot_pixel(A.slot_base + slot, A.W, A.H, x, y)) return;
     const uint32_t p = y * A.W + x;
-    // the last pass's samples join the sum of the earlier ones, in order
-    const uint32_t passes = (A.spp + A.S - 1) / A.S, sk = A.spp - (passes - 1) * A.S;
-    float4 ps = passes == 1 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : A.psum[p];
-    for (uint32_t sl = 0; sl < sk; ++sl) {
-        const float4 s1 = A.st1[sl * A.npix + p];
-        ps = make_float4(ps.x + s1.x, ps.y + s1.y, ps.z + s1.z, 0.0f);
+    const uint32_t passes = (A.spp + A.S - 1) / A.S;
+    float pc0, pc1, pc2;
+    bool bad_sum = false;
+    if (A.streams) {
+        // the last pass's samples join the order-free sum of the earlier ones
+        FixSum fs;
+        if (passes == 1) fix_clear(fs);
+        else fs = A.fsum[p];
+        fold_pass(A, passes - 1, p, fs);
+        pc0 = fix_value(fs.c[0]);
+        pc1 = fix_value(fs.c[1]);
+        pc2 = fix_value(fs.c[2]);
+        bad_sum = fs.bad != 0;
+    } else {
+        // one sample per pass: the last one joins the float sum
+        const float4 s1 = A.st1[p];
+        const float4 ps = passes == 1 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : A.psum[p];
+        pc0 = ps.x + s1.x;
+        pc1 = ps.y + s1.y;
+        pc2 = ps.z + s1.z;
     }
-    float pc0 = ps.x, pc1 = ps.y, pc2 = ps.z;
     const float inv = 1.0f / (float)A.spp;   // pathtrace.rgen:88
     pc0 = inv * pc0;
     pc1 = inv * pc1;
@@ -534,7 +555,7 @@ __global__ void k_pt_finalize(WfArgs A)
     const float4 old = *px;
     const float alpha = A.fc->alpha;
     float m0 = mixf(old.x, pc0, alpha), m1 = mixf(old.y, pc1, alpha), m2 = mixf(old.z, pc2, alpha);
-    const bool bad = isnan(m0) || isnan(m1) || isnan(m2) || isinf(m0) || isinf(m1) || isinf(m2);
+    const bool bad = bad_sum || isnan(m0) || isnan(m1) || isnan(m2) || isinf(m0) || isinf(m1) || isinf(m2);
     if (bad) {   // :94-99
         m0 = old.x;
         m1 = old.y;
@@ -724,7 +745,7 @@ int launch_pathtrace_passes(hipStream_t s, const WfArgs &a0, Timer *timer, const
     };
     const bool nrc = a.nrc.enabled != 0;
     for (uint32_t pass = 0; pass < passes; ++pass) {
-        if (a.S > 1 && pass > 0)
+        if (a.streams && pass > 0)
             timed(SOLHIP_KERNEL_PT_GENERATE,
                   [&] { hipLaunchKernelGGL(k_pt_fold, dim3(fold_blocks), dim3(blk), 0, s, a, pass - 1); });
         timed(SOLHIP_KERNEL_PT_GENERATE,

@@ -30,8 +30,10 @@ def draw_requests(n, rate, seed=3, resolutions=RESOLUTIONS, spp_max=256):
     return reqs
 
 
-def run_load(ctx, gpu_scenes, scene_data, reqs, hbm_budget=8 << 30, max_jobs=64, steps_per_poll=2):
-    """serves reqs (draw_requests) on a fresh server; returns a stats dict"""
+def run_load(ctx, gpu_scenes, scene_data, reqs, hbm_budget=8 << 30, max_jobs=64, steps_per_poll=2, streams=0):
+    """serves reqs (draw_requests) on a fresh server; returns a stats dict.
+    streams = 1: per-sample random streams, so a job may run on several
+    slots per pixel (the server picks how many)"""
     srv = api.Server(ctx, hbm_budget, max_jobs)
     sids = [srv.scene(g) for g in gpu_scenes]
     bufs = {}
@@ -49,6 +51,7 @@ def run_load(ctx, gpu_scenes, scene_data, reqs, hbm_budget=8 << 30, max_jobs=64,
             bufs[k] = ctx.buffer(W * H * 16)   # each job its own image
             spec = api.spec_default(api.PROG_PATHTRACE)
             spec.enable_sky, spec.spp_per_frame, spec.max_depth, spec.rr_start, spec.nee = 2, r["spp"], r["depth"], 3, 1
+            spec.sample_streams = streams
             jid = srv.submit(sids[si], W, H, spec, scene_data[si].uniforms(W, H, k), bufs[k], accum_start=k)
             jobs.append((jid, r))
             k += 1

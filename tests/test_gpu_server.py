@@ -14,10 +14,10 @@ from solhip import scenes
 pytestmark = pytest.mark.gpu
 
 
-def _spec(sky=0, spp=3, depth=4, rr=0, nee=0, streams=0):
+def _spec(sky=0, spp=3, depth=4, rr=0, nee=0, streams=0, lanes=0):
     s = solhip.spec_default(solhip.PROG_PATHTRACE)
     s.enable_sky, s.spp_per_frame, s.max_depth, s.rr_start, s.nee = sky, spp, depth, rr, nee
-    s.sample_streams = streams
+    s.sample_streams, s.wf_samples_per_pass = streams, lanes
     return s
 
 
@@ -38,7 +38,13 @@ JOBS = [  # (scene, W, H, spec kwargs, frame)
     ("instances", 33, 20, dict(sky=2, spp=4, depth=8, nee=1, rr=3), 3),
     ("cornell", 64, 48, dict(spp=1, depth=5), 4),
     ("instances", 30, 22, dict(sky=2, spp=4, depth=6, nee=1, rr=2, streams=1), 5),
+    # per-sample streams on several slots per pixel: 3 lanes for 5 samples
+    # (lanes with 2, 2 and 1 samples), and the default lanes for 40 samples (3)
+    ("cornell", 29, 21, dict(spp=5, depth=5, nee=1, rr=2, streams=1, lanes=3), 6),
+    ("instances", 24, 20, dict(sky=2, spp=40, depth=4, nee=1, rr=3, streams=1), 7),
 ]
+SLOT_BYTES = 220
+LANES = [3 if kw.get("lanes") == 3 else (3 if kw.get("spp", 0) == 40 else 1) for _, _, _, kw, _ in JOBS]
 
 
 def _run_jobs(ctx, budget, max_jobs=64):
@@ -80,8 +86,8 @@ def test_budget_for_one_job_at_a_time(ctx):
     """the budget holds the largest job only: jobs run one after another in
     submission order and still give the same images"""
     print("server: tight budget", flush=True)
-    largest = max(W * H for _, W, H, _, _ in JOBS)
-    srv, sds, gss, jobs = _run_jobs(ctx, largest * 210 + 64 * 210)
+    largest = max(W * H * ln for (_, W, H, _, _), ln in zip(JOBS, LANES))
+    srv, sds, gss, jobs = _run_jobs(ctx, largest * SLOT_BYTES + 64 * SLOT_BYTES)
     sts = [srv.job(j[0]) for j in jobs]
     assert all(s.state == solhip.JOB_DONE for s in sts)
     for a, b in zip(sts, sts[1:]):
