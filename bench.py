@@ -67,9 +67,10 @@ def parse_args():
                     choices=["ao", "pathtrace-c0", "pathtrace-c2", "server-c3", "cache-c4"])
     ap.add_argument("--jobs", type=int, default=40, help="server-c3: requests")
     ap.add_argument("--rate", type=float, default=200.0, help="server-c3: requests per second")
-    ap.add_argument("--streams", type=int, default=1, choices=(0, 1),
-                    help="pathtrace-c0/c2, server-c3: 1 = a random stream per sample, 0 = one per pixel "
-                         "(pathtrace.rgen)")
+    ap.add_argument("--streams", type=int, default=None, choices=(0, 1),
+                    help="1 = a random stream per sample, 0 = one per pixel (pathtrace.rgen); default 1 for "
+                         "pathtrace-c0/c2, 0 for server-c3 (its jobs already fill the GPU; extra slots per pixel "
+                         "only delay admissions)")
     ap.add_argument("--per-pass", type=int, default=0, help="pathtrace-c0/c2: samples per pass (0 = library default)")
     ap.add_argument("--slices", type=int, default=0, help="pathtrace-c0/c2: image slices (0 = library default)")
     ap.add_argument("--any-hit", action="store_true",
@@ -269,7 +270,7 @@ def bench_pathtrace(a):
     # one random stream per sample (sample_streams = 1) lets a pass trace many
     # samples of every pixel at once; 0 keeps pathtrace.rgen's one stream per
     # pixel, whose samples must follow one another
-    spec.sample_streams = a.streams
+    spec.sample_streams = 1 if a.streams is None else a.streams
     spec.wf_samples_per_pass = a.per_pass
     spec.wf_slices = a.slices
     prog = ctx.program(solhip.PROG_PATHTRACE, spec)
@@ -309,7 +310,7 @@ def bench_pathtrace(a):
                                 "depth 8, Russian roulette from bounce 3, next-event estimation (no emitters), "
                                 "gradient sky, all BLAS + TLAS rebuilt per frame" % sd.tri_count),
                    "resolution": [W, H], "spp": spec.spp_per_frame, "max_depth": spec.max_depth,
-                   "sample_streams": a.streams, "parallelism": "single GPU"},
+                   "sample_streams": spec.sample_streams, "parallelism": "single GPU"},
         "extra": {"rays_per_frame": int(st.rays / a.steps), "hipgraph": True,
                   "samples_per_pass": a.per_pass or "default", "slices": a.slices or "default"},
     }
@@ -327,16 +328,17 @@ def bench_server(a):
     gss = [solhip.GpuScene(ctx, sd) for sd in sds]
     ctx.sync()
     warm = serve.draw_requests(4, 1000.0, seed=1, resolutions=[(256, 256)], spp_max=4)
-    serve.run_load(ctx, gss, sds, warm, streams=a.streams)
+    streams = 0 if a.streams is None else a.streams
+    serve.run_load(ctx, gss, sds, warm, streams=streams)
     reqs = serve.draw_requests(a.jobs, a.rate, seed=3)
-    stats = serve.run_load(ctx, gss, sds, reqs, streams=a.streams)
+    stats = serve.run_load(ctx, gss, sds, reqs, streams=streams)
     result = {
         "metric": "render-request latency on 1 GPU", "value": round(stats["latency_ms_p50"], 2), "unit": "ms (p50)",
         "n_gpus": 1, "steps": a.jobs, "warmup": 4, "ms_per_step": round(stats["makespan_s"] * 1e3 / a.jobs, 3),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "render server, Poisson %.0f req/s, %d requests over Cornell / 2.1M-triangle "
                                "heightfield / 10k instances, resolutions 256^2..1080p, spp log-uniform 1-256, "
-                               "depth 4-8, NEE + roulette" % (a.rate, a.jobs), "sample_streams": a.streams,
+                               "depth 4-8, NEE + roulette" % (a.rate, a.jobs), "sample_streams": streams,
                    "parallelism": "single GPU"},
         "extra": stats,
     }

@@ -413,6 +413,7 @@ constexpr uint64_t kSlotBytes = 48 + 32 + 4 + 64 + 20 + 48 + 4;
 // about kJobSlots slots for the job
 constexpr uint32_t kLaneSamples = 16;
 constexpr uint64_t kJobSlots = 4ull << 20;
+constexpr uint32_t kLaneShare = 4;
 
 struct SrvScene {
     SceneView sv;
@@ -427,7 +428,8 @@ struct SrvJob {
     uint32_t state = SOLHIP_JOB_QUEUED;
     double t_submit = 0, t_admit = 0, t_done = 0;
     uint32_t base = 0, npix = 0, jidx = 0;
-    uint32_t lanes = 1, slots = 0;   // slots per pixel, slots of the job (lanes * npix)
+    uint32_t lanes_max = 1;          // slots per pixel the job may use (per-sample streams)
+    uint32_t lanes = 1, slots = 0;   // slots per pixel, slots of the job (lanes * npix), set at admission
 };
 
 struct Server : Obj {
@@ -521,8 +523,19 @@ struct Server : Obj {
             uint32_t jidx = 0;
             while (jidx < max_jobs && running_by_idx[jidx]) ++jidx;
             if (jidx == max_jobs) return 0;
+            // slots per pixel: what the job asks for, but extra lanes take at
+            // most 1/kLaneShare of the largest free range, so that jobs
+            // behind it are not kept waiting; one lane if that is all that fits
+            uint64_t lanes = jb.lanes_max, largest = 0;
+            for (const auto &r : free_ranges) largest = std::max<uint64_t>(largest, r.second);
+            lanes = std::max<uint64_t>(1, std::min<uint64_t>(lanes, largest / ((uint64_t)kLaneShare * jb.npix)));
             uint32_t base = 0;
-            if (!alloc_range(jb.slots, base)) return 0;
+            if (!alloc_range((uint32_t)(lanes * jb.npix), base)) {
+                if (lanes == 1 || !alloc_range(jb.npix, base)) return 0;
+                lanes = 1;
+            }
+            jb.lanes = (uint32_t)lanes;
+            jb.slots = jb.lanes * jb.npix;
             const SrvScene *sc = scenes[jb.desc.scene];
             Buffer *acc = get<Buffer>(jb.desc.accum, Kind::Buffer);
             Buffer *out = jb.desc.out_rgba8 ? get<Buffer>(jb.desc.out_rgba8, Kind::Buffer) : nullptr;
@@ -766,9 +779,8 @@ int32_t solhip_server_submit(solhip_ctx ctx, solhip_server h, const solhip_job_d
         if (lanes == 0)
             lanes = std::min<uint64_t>((spp + kLaneSamples - 1) / kLaneSamples, std::max<uint64_t>(1, kJobSlots / npix));
         lanes = std::max<uint64_t>(1, std::min<uint64_t>({lanes, spp, sv->max_slots / npix}));
-        jb.lanes = (uint32_t)lanes;
+        jb.lanes_max = (uint32_t)lanes;
     }
-    jb.slots = jb.lanes * jb.npix;
     jb.t_submit = now_ms(sv->t0);
     const uint64_t id = sv->next_id++;
     sv->all[id] = jb;
