@@ -358,15 +358,17 @@ def bench_cache(a):
     ctx = solhip.Context(0)
     gs = solhip.GpuScene(ctx, sd)
 
-    def spec(nrc, spp):
+    def spec(nrc, spp, streams=0):
         s = solhip.spec_default(solhip.PROG_PATHTRACE)
         s.enable_sky, s.spp_per_frame, s.max_depth, s.rr_start, s.nee, s.nrc = 2, spp, 8, 3, 1, nrc
+        s.sample_streams = streams
         return s
 
     acc = ctx.upload(np.zeros((H, W, 4), np.float32))
     img = solhip.Images(acc.h, 0, 0, 0, 0, 0)
-    # reference: 4096 spp of the plain tracer (frames of 64 spp, running mean)
-    pr = ctx.program(solhip.PROG_PATHTRACE, spec(0, 64))
+    # reference: 4096 spp of the plain tracer (frames of 64 spp with per-sample
+    # streams, running mean; the same expectation as the estimators compared)
+    pr = ctx.program(solhip.PROG_PATHTRACE, spec(0, 64, streams=1))
     for f in range(64):
         ctx.trace(pr, gs.tlas, gs.tables, sd.uniforms(W, H, 100000 + f), 100000, img, W, H)
     ref = acc.download(np.float32).reshape(H, W, 4)[..., :3].astype(np.float64)
