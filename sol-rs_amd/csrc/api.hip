@@ -94,6 +94,7 @@ SceneView view_of(const Tlas *t)
 {
     SceneView v;
     v.stack_depth = std::min<uint32_t>(stack_depth_of(t), (uint32_t)kMaxStack);
+    v.single = t->bvh.prim_count == 1 ? 1u : 0u;
     if (t->bvh.prim_count == 0) {
         v.tnodes = nullptr;
         v.insts = nullptr;

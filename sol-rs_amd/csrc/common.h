@@ -71,6 +71,7 @@ struct SceneView {
     const InstRec *insts;       // TLAS leaf records
     const uint32_t *inst_meta;  // index | mask << 24
     uint32_t stack_depth;       // LDS stack entries per lane (>= TLAS + BLAS depth)
+    uint32_t single;            // 1: the TLAS holds one instance; rays enter it without the root test
 };
 
 struct Counters {

@@ -332,8 +332,9 @@ SOLHIP_HD Hit trace_ray(const SceneView sv, V3 wo, V3 wd, float tmin, float tmax
 
         const V3 winv = safe_inv(wd);
         const uint32_t woct = octant(wd);
-        uint32_t ng_base = 0, ng_hits = 1u << woct, ng_imask = 1u;
-        uint32_t tg_base = 0, tg_mask = 0;
+        // the TLAS root, or, with one instance, its leaf (record 0) at once
+        uint32_t ng_base = 0, ng_hits = sv.single ? 0u : 1u << woct, ng_imask = 1u;
+        uint32_t tg_base = 0, tg_mask = sv.single ? 1u : 0u;
         while (true) {
             if (tg_mask) {
                 // instance leaf: transform the ray and trace its BLAS

@@ -47,11 +47,12 @@ SOLHIP_HD void trav_init(TravState &s, const SceneView &sv, V3 wo, V3 wd, float 
     s.inst = 0;
     s.nodes = nullptr;
     s.tris = nullptr;
+    // the TLAS root, or, with one instance, its leaf (record 0) at once
     s.ng_base = 0;
-    s.ng_hits = sv.tnodes ? (1u << s.oct) : 0u;
+    s.ng_hits = sv.tnodes && !sv.single ? (1u << s.oct) : 0u;
     s.ng_imask = 1u;
     s.tg_base = 0;
-    s.tg_mask = 0;
+    s.tg_mask = sv.tnodes && sv.single ? 1u : 0u;
     s.sp = 0;
     s.base = 0;
     s.best.t = tmax;

@@ -217,6 +217,7 @@ void hc_trace(void *h, const float *rays, uint32_t n, uint32_t any, void *hits, 
     SceneView sv;
     sv.tnodes = s->insts.empty() ? nullptr : s->tlas.nodes.data();
     sv.insts = s->insts.data();
+    sv.single = s->insts.size() == 1 ? 1u : 0u;
     sv.inst_meta = s->meta.data();
     uint32_t max_blas_depth = 0;
     for (const auto &b : s->blas) max_blas_depth = std::max(max_blas_depth, b.bvh.depth);
@@ -244,6 +245,7 @@ void hc_trace_steps(void *h, const float *rays, uint32_t n, uint32_t any, void *
     SceneView sv;
     sv.tnodes = s->insts.empty() ? nullptr : s->tlas.nodes.data();
     sv.insts = s->insts.data();
+    sv.single = s->insts.size() == 1 ? 1u : 0u;
     sv.inst_meta = s->meta.data();
     uint32_t max_blas_depth = 0;
     for (const auto &b : s->blas) max_blas_depth = std::max(max_blas_depth, b.bvh.depth);
