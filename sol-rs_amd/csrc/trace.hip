@@ -3,6 +3,7 @@
 
 #include "kernels.h"
 #include "traverse.h"
+#include "traverse_step.h"
 
 namespace solhip {
 
@@ -46,8 +47,14 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_rays(SceneView sv, const 
     if (active) {
         const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
         const float4 a = rp[0], b = rp[1];
-        Hit h = trace_ray<ANY, COUNT>(sv, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, 0xFFu, stack + threadIdx.x,
-                                      kTraceBlock, tc);
+        // the step traversal run to the end (trace_ray's visit order, hits
+        // and counts, in fewer registers); sv.stack_depth includes its two
+        // extra entries
+        TravState st;
+        trav_init(st, sv, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w);
+        while (!trav_step<ANY, COUNT>(st, sv, 0xFFu, stack + threadIdx.x, kTraceBlock, tc)) {
+        }
+        const Hit h = trav_result(st);
         solhip_hit o;
         o.t = h.t;
         o.u = h.u;
@@ -71,16 +78,18 @@ int launch_trace_rays(hipStream_t s, const SceneView &sv, const solhip_ray *rays
     if (n == 0) return 0;
     if (timer) timer->begin(s, SOLHIP_KERNEL_TRACE_RAYS);
     const dim3 grid((n + kTraceBlock - 1) / kTraceBlock), block(kTraceBlock);
-    const size_t lds = (size_t)sv.stack_depth * sizeof(uint2) * kTraceBlock;
+    SceneView v = sv;
+    v.stack_depth += 2;
+    const size_t lds = (size_t)v.stack_depth * sizeof(uint2) * kTraceBlock;
     const bool any = flags & SOLHIP_TRACE_TERMINATE_ON_FIRST_HIT, count = flags & SOLHIP_TRACE_COUNT;
     if (any && count)
-        hipLaunchKernelGGL((k_trace_rays<true, true>), grid, block, lds, s, sv, rays, hits, n, counters);
+        hipLaunchKernelGGL((k_trace_rays<true, true>), grid, block, lds, s, v, rays, hits, n, counters);
     else if (any)
-        hipLaunchKernelGGL((k_trace_rays<true, false>), grid, block, lds, s, sv, rays, hits, n, counters);
+        hipLaunchKernelGGL((k_trace_rays<true, false>), grid, block, lds, s, v, rays, hits, n, counters);
     else if (count)
-        hipLaunchKernelGGL((k_trace_rays<false, true>), grid, block, lds, s, sv, rays, hits, n, counters);
+        hipLaunchKernelGGL((k_trace_rays<false, true>), grid, block, lds, s, v, rays, hits, n, counters);
     else
-        hipLaunchKernelGGL((k_trace_rays<false, false>), grid, block, lds, s, sv, rays, hits, n, counters);
+        hipLaunchKernelGGL((k_trace_rays<false, false>), grid, block, lds, s, v, rays, hits, n, counters);
     if (timer) timer->end(s, SOLHIP_KERNEL_TRACE_RAYS);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
