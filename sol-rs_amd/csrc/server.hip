@@ -96,6 +96,36 @@ __device__ __forceinline__ uint32_t srv_wave_append(uint32_t *counter, bool pred
     return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
 }
 
+// shade and retire run blocks of kSrvShadeWaves waves and append to the
+// queues once per block: three appends per pass over one counter each for
+// all rays of every job made the per-wave atomics on those few addresses the
+// largest cost of a step
+constexpr uint32_t kSrvShadeWaves = 8;
+constexpr uint32_t kSrvShadeBlock = 64 * kSrvShadeWaves;
+
+// Block-wide append: every thread of the block calls it (block-uniform
+// control flow); one atomic per block.  lds: kSrvShadeWaves + 1 words.
+__device__ __forceinline__ uint32_t srv_block_append(uint32_t *counter, bool pred, uint32_t *lds)
+{
+    const uint64_t m = __ballot(pred);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (lane == 0) lds[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (uint32_t w = 0; w < kSrvShadeWaves; ++w) {
+            const uint32_t c = lds[w];
+            lds[w] = total;
+            total += c;
+        }
+        lds[kSrvShadeWaves] = total ? atomicAdd(counter, total) : 0u;
+    }
+    __syncthreads();
+    const uint32_t base = lds[kSrvShadeWaves] + lds[wave];
+    __syncthreads();   // lds is reused by the next call
+    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
 // pathtrace.rgen:35 for this job's camera
 __device__ __forceinline__ float job_tmin(const JobDev &J)
 {
@@ -246,12 +276,13 @@ __global__ void __launch_bounds__(kSrvBlock) SOLHIP_SRV_OCCUPANCY k_srv_shadow(S
     srv_trace<true>(A, A.sq, A.counts[2], A.heads + kSrvParts * kHeadStride, SOLHIP_KERNEL_SRV_SHADOW);
 }
 
-__global__ void __launch_bounds__(kSrvBlock) k_srv_shade(SrvArgs A, uint32_t cur)
+__global__ void __launch_bounds__(kSrvShadeBlock) k_srv_shade(SrvArgs A, uint32_t cur)
 {
+    __shared__ uint32_t lds[kSrvShadeWaves + 1];
     const uint32_t cnt = A.counts[cur];
-    const uint32_t lane = threadIdx.x & 63u, nthreads = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < cnt; base += nthreads) {
-        const uint32_t i = base + lane;
+    const uint32_t nthreads = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < cnt; base += nthreads) {
+        const uint32_t i = base + threadIdx.x;
         ShadeOut o;
         bool ended = false;
         uint32_t slot = 0;
@@ -275,28 +306,29 @@ __global__ void __launch_bounds__(kSrvBlock) k_srv_shade(SrvArgs A, uint32_t cur
             A.st1[slot] = s1;
             ended = !o.cont;
         }
-        const uint32_t s_next = srv_wave_append(A.counts + (cur ^ 1u), o.cont);
+        const uint32_t s_next = srv_block_append(A.counts + (cur ^ 1u), o.cont, lds);
         if (o.cont) A.q[cur ^ 1u][s_next] = o.nr;
-        const uint32_t s_sh = srv_wave_append(A.counts + 2, o.shadow);
+        const uint32_t s_sh = srv_block_append(A.counts + 2, o.shadow, lds);
         if (o.shadow) {
             A.sq[s_sh] = o.sr;
             A.scontrib[s_sh] = o.sc;
         }
-        const uint32_t s_rt = srv_wave_append(A.counts + 3, ended);
+        const uint32_t s_rt = srv_block_append(A.counts + 3, ended, lds);
         if (ended) A.retire[s_rt] = slot;
     }
 }
 
-__global__ void __launch_bounds__(kSrvBlock) k_srv_retire(SrvArgs A, uint32_t cur)
+__global__ void __launch_bounds__(kSrvShadeBlock) k_srv_retire(SrvArgs A, uint32_t cur)
 {
+    __shared__ uint32_t lds[kSrvShadeWaves + 1];
     if (blockIdx.x == 0) {
         // this step's extend and shadow are complete
         for (uint32_t k = threadIdx.x; k < 2 * kSrvParts; k += blockDim.x) A.heads[k * kHeadStride] = 0;
     }
     const uint32_t cnt = A.counts[3];
-    const uint32_t lane = threadIdx.x & 63u, nthreads = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < cnt; base += nthreads) {
-        const uint32_t i = base + lane;
+    const uint32_t nthreads = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < cnt; base += nthreads) {
+        const uint32_t i = base + threadIdx.x;
         bool again = false;
         QRay r;
         if (i < cnt) {
@@ -382,7 +414,7 @@ __global__ void __launch_bounds__(kSrvBlock) k_srv_retire(SrvArgs A, uint32_t cu
                 atomicAdd(A.done + job, 1u);
             }
         }
-        const uint32_t at = srv_wave_append(A.counts + (cur ^ 1u), again);
+        const uint32_t at = srv_block_append(A.counts + (cur ^ 1u), again, lds);
         if (again) A.q[cur ^ 1u][at] = r;
     }
 }
@@ -593,7 +625,8 @@ struct Server : Obj {
     {
         const SrvArgs a = args();
         const uint32_t persist = cx->ncu * 4 * SOLHIP_SRV_WAVES;
-        const uint32_t shade_blocks = std::min<uint32_t>((max_slots + kSrvBlock - 1) / kSrvBlock, cx->ncu * 32);
+        const uint32_t shade_blocks =
+            std::min<uint32_t>((max_slots + kSrvShadeBlock - 1) / kSrvShadeBlock, cx->ncu * 4);
         const size_t lds = (size_t)(max_stack + 2) * sizeof(uint2) * kSrvBlock;
         Timer *timer = cx->timer_or_null();
         auto timed = [&](int id, auto &&launch) {
@@ -604,11 +637,11 @@ struct Server : Obj {
         timed(SOLHIP_KERNEL_SRV_EXTEND,
               [&] { hipLaunchKernelGGL(k_srv_extend, dim3(persist), dim3(kSrvBlock), lds, stream, a, cur); });
         timed(SOLHIP_KERNEL_SRV_SHADE,
-              [&] { hipLaunchKernelGGL(k_srv_shade, dim3(shade_blocks), dim3(kSrvBlock), 0, stream, a, cur); });
+              [&] { hipLaunchKernelGGL(k_srv_shade, dim3(shade_blocks), dim3(kSrvShadeBlock), 0, stream, a, cur); });
         timed(SOLHIP_KERNEL_SRV_SHADOW,
               [&] { hipLaunchKernelGGL(k_srv_shadow, dim3(persist), dim3(kSrvBlock), lds, stream, a); });
         timed(SOLHIP_KERNEL_SRV_RETIRE,
-              [&] { hipLaunchKernelGGL(k_srv_retire, dim3(shade_blocks), dim3(kSrvBlock), 0, stream, a, cur); });
+              [&] { hipLaunchKernelGGL(k_srv_retire, dim3(shade_blocks), dim3(kSrvShadeBlock), 0, stream, a, cur); });
         cur ^= 1u;
         return srv_status("server step");
     }
