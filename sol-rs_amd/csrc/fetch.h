@@ -1,4 +1,5 @@
-// fetch.h -- work distribution of the persistent traversal kernels.
+// fetch.h -- work distribution of the persistent traversal kernels, and the
+// block-wide queue append of the kernels that fill their queues.
 //
 // The items [0, total) of one launch are split into P contiguous partitions,
 // each with its own fetch head; heads sit kHeadStride words (128 B) apart so
@@ -60,5 +61,31 @@ struct PartFetch {
         return base;
     }
 };
+
+// Block-wide append of one item per thread that has one: every thread of the
+// block calls it with block-uniform control flow; one atomic per block (a
+// producer kernel that appended once per wave put every wave of the GPU on
+// one counter address, which limited it).  lds: WAVES + 1 words.
+template <uint32_t WAVES>
+__device__ __forceinline__ uint32_t block_append(uint32_t *counter, bool pred, uint32_t *lds)
+{
+    const uint64_t m = __ballot(pred);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (lane == 0) lds[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (uint32_t w = 0; w < WAVES; ++w) {
+            const uint32_t c = lds[w];
+            lds[w] = total;
+            total += c;
+        }
+        lds[WAVES] = total ? atomicAdd(counter, total) : 0u;
+    }
+    __syncthreads();
+    const uint32_t base = lds[WAVES] + lds[wave];
+    __syncthreads();   // lds is reused by the next call
+    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
 
 }  // namespace solhip

@@ -103,29 +103,6 @@ __device__ __forceinline__ uint32_t srv_wave_append(uint32_t *counter, bool pred
 constexpr uint32_t kSrvShadeWaves = 8;
 constexpr uint32_t kSrvShadeBlock = 64 * kSrvShadeWaves;
 
-// Block-wide append: every thread of the block calls it (block-uniform
-// control flow); one atomic per block.  lds: kSrvShadeWaves + 1 words.
-__device__ __forceinline__ uint32_t srv_block_append(uint32_t *counter, bool pred, uint32_t *lds)
-{
-    const uint64_t m = __ballot(pred);
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    if (lane == 0) lds[wave] = (uint32_t)__popcll(m);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t total = 0;
-        for (uint32_t w = 0; w < kSrvShadeWaves; ++w) {
-            const uint32_t c = lds[w];
-            lds[w] = total;
-            total += c;
-        }
-        lds[kSrvShadeWaves] = total ? atomicAdd(counter, total) : 0u;
-    }
-    __syncthreads();
-    const uint32_t base = lds[kSrvShadeWaves] + lds[wave];
-    __syncthreads();   // lds is reused by the next call
-    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-}
-
 // pathtrace.rgen:35 for this job's camera
 __device__ __forceinline__ float job_tmin(const JobDev &J)
 {
@@ -306,14 +283,14 @@ __global__ void __launch_bounds__(kSrvShadeBlock) k_srv_shade(SrvArgs A, uint32_
             A.st1[slot] = s1;
             ended = !o.cont;
         }
-        const uint32_t s_next = srv_block_append(A.counts + (cur ^ 1u), o.cont, lds);
+        const uint32_t s_next = block_append<kSrvShadeWaves>(A.counts + (cur ^ 1u), o.cont, lds);
         if (o.cont) A.q[cur ^ 1u][s_next] = o.nr;
-        const uint32_t s_sh = srv_block_append(A.counts + 2, o.shadow, lds);
+        const uint32_t s_sh = block_append<kSrvShadeWaves>(A.counts + 2, o.shadow, lds);
         if (o.shadow) {
             A.sq[s_sh] = o.sr;
             A.scontrib[s_sh] = o.sc;
         }
-        const uint32_t s_rt = srv_block_append(A.counts + 3, ended, lds);
+        const uint32_t s_rt = block_append<kSrvShadeWaves>(A.counts + 3, ended, lds);
         if (ended) A.retire[s_rt] = slot;
     }
 }
@@ -414,7 +391,7 @@ __global__ void __launch_bounds__(kSrvShadeBlock) k_srv_retire(SrvArgs A, uint32
                 atomicAdd(A.done + job, 1u);
             }
         }
-        const uint32_t at = srv_block_append(A.counts + (cur ^ 1u), again, lds);
+        const uint32_t at = block_append<kSrvShadeWaves>(A.counts + (cur ^ 1u), again, lds);
         if (again) A.q[cur ^ 1u][at] = r;
     }
 }

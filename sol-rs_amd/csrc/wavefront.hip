@@ -291,13 +291,21 @@ __global__ void SOLHIP_WF_OCCUPANCY k_pt_shadow(WfArgs A, uint32_t b)
                              SOLHIP_KERNEL_PT_SHADOW);
 }
 
-__global__ void k_pt_shade(WfArgs A, uint32_t b)
+// The shade kernel runs blocks of WAVES waves and appends to its queues once
+// per block.  With shadow rays (two appends per ray) 8-wave blocks: the
+// per-wave atomics on the two counters limited the Cornell box (3599 ->
+// 4223 Mrays/s); without them 2-wave blocks: larger blocks wait longer at the
+// append barriers on scattered scenes (10k instances: 1259 with 1 wave, 1277
+// with 2, 1225 with 8).
+template <uint32_t WAVES>
+__global__ void __launch_bounds__(64 * WAVES) k_pt_shade(WfArgs A, uint32_t b)
 {
+    __shared__ uint32_t lds[WAVES + 1];
     const uint32_t cnt = A.counts[b & 1];
     const QRay *qin = A.q[b & 1];
     QRay *qout = A.q[(b + 1) & 1];
     uint32_t *cout = A.counts + ((b + 1) & 1);
-    const uint32_t lane = threadIdx.x & 63u, nthreads = gridDim.x * blockDim.x;
+    const uint32_t nthreads = gridDim.x * blockDim.x;
     ShadeParams P;
     P.tb = A.tb;
     P.enable_sky = A.enable_sky;
@@ -308,8 +316,8 @@ __global__ void k_pt_shade(WfArgs A, uint32_t b)
     P.total = A.nlights ? A.lights[A.nlights - 1].cdf : 0.0f;
     P.nee = A.nee && A.nlights > 0 && P.total > 0.0f;
     P.rmax = A.rmax;
-    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < cnt; base += nthreads) {
-        const uint32_t i = base + lane;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < cnt; base += nthreads) {
+        const uint32_t i = base + threadIdx.x;
         ShadeOut o;
         if (i < cnt) {
             const QRay r = qin[i];
@@ -321,9 +329,9 @@ __global__ void k_pt_shade(WfArgs A, uint32_t b)
                 A.st1[p] = s1;
             }
         }
-        const uint32_t s_next = wave_append(cout, o.cont);
+        const uint32_t s_next = block_append<WAVES>(cout, o.cont, lds);
         if (o.cont) qout[s_next] = o.nr;
-        const uint32_t s_sh = wave_append(A.counts + 2, o.shadow);
+        const uint32_t s_sh = block_append<WAVES>(A.counts + 2, o.shadow, lds);
         if (o.shadow) {
             A.sq[s_sh] = o.sr;
             A.scontrib[s_sh] = o.sc;
@@ -737,6 +745,9 @@ int launch_pathtrace_passes(hipStream_t s, const WfArgs &a0, Timer *timer, const
     const uint32_t gen_blocks = (std::max(a.S * a.nslots, nheads * kWfParts) + blk - 1) / blk;
     const uint32_t fold_blocks = (a.nslots + blk - 1) / blk;
     const uint32_t passes = (a.spp + a.S - 1) / a.S;
+    auto shade_grid = [&](uint32_t waves) {
+        return std::max(1u, (uint32_t)(((uint64_t)a.shade_blocks * blk + 64 * waves - 1) / (64 * waves)));
+    };
     const bool shadows = a.nee && a.nlights > 0;
     auto timed = [&](int id, auto &&launch) {
         if (timer) timer->begin(s, id);
@@ -762,7 +773,8 @@ int launch_pathtrace_passes(hipStream_t s, const WfArgs &a0, Timer *timer, const
             });
             timed(SOLHIP_KERNEL_PT_SHADE, [&] {
                 if (nrc) hipLaunchKernelGGL(k_pt_shade_nrc, dim3(a.shade_blocks), dim3(blk), 0, s, a, b);
-                else hipLaunchKernelGGL(k_pt_shade, dim3(a.shade_blocks), dim3(blk), 0, s, a, b);
+                else if (shadows) hipLaunchKernelGGL(k_pt_shade<8>, dim3(shade_grid(8)), dim3(512), 0, s, a, b);
+                else hipLaunchKernelGGL(k_pt_shade<2>, dim3(shade_grid(2)), dim3(128), 0, s, a, b);
             });
             if (nrc && b == 1 && hook) {
                 int rc = hook->after_query_shade(hook->user, s);
