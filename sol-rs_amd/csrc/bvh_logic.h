@@ -21,14 +21,19 @@ SOLHIP_HD uint32_t expand10(uint32_t v)
     return v;
 }
 
-// 30-bit Morton code of the box centroid within the centroid bounds
+// 30-bit Morton code of the box centroid within the centroid bounds.  All
+// three axes share one scale, 1024 cells over the largest extent: scaling each
+// axis to its own extent would give a thin axis (the height of a terrain) as
+// many top-level splits as the wide ones, and those slabs cut across the whole
+// surface (a 1024^2 terrain: 16.6 -> 12.8 node tests per primary ray).
 SOLHIP_HD uint32_t morton30(const float lo[3], const float hi[3], const float blo[3], const float bhi[3])
 {
+    const float ext = fmaxf(fmaxf(bhi[0] - blo[0], bhi[1] - blo[1]), bhi[2] - blo[2]);
+    const float scale = ext > 0.0f ? 1024.0f / ext : 0.0f;
     uint32_t code = 0;
     for (int a = 0; a < 3; ++a) {
         const float c = 0.5f * lo[a] + 0.5f * hi[a];
-        const float ext = bhi[a] - blo[a];
-        const float q = ext > 0.0f ? (c - blo[a]) * (1024.0f / ext) : 0.0f;
+        const float q = (c - blo[a]) * scale;
         const uint32_t qi = q <= 0.0f ? 0u : (q >= 1023.0f ? 1023u : (uint32_t)q);
         code |= expand10(qi) << (2 - a);
     }
