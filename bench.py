@@ -131,7 +131,8 @@ def main():
     sd = scenes.heightfield(a.grid)
     ctx = solhip.Context(0)
     t0 = time.perf_counter()
-    gs = solhip.GpuScene(ctx, sd)
+    # one-time build: binned SAH (the PREFER_FAST_TRACE build of BLAS::new)
+    gs = solhip.GpuScene(ctx, sd, build_flags=solhip.BUILD_PREFER_FAST_TRACE)
     ctx.sync()
     upload_build_s = time.perf_counter() - t0
     # BVH build alone (rebuild in place on the resident vertex buffers)
@@ -260,7 +261,8 @@ def bench_pathtrace(a):
     sd = scenes.cornell_box() if c0 else scenes.instanced(10000)
     W, H = (512, 512) if c0 else (1920, 1080)
     ctx = solhip.Context(0)
-    gs = solhip.GpuScene(ctx, sd)
+    # C0 is static: binned SAH; C2 rebuilds every BLAS each frame: Morton order
+    gs = solhip.GpuScene(ctx, sd, build_flags=solhip.BUILD_PREFER_FAST_TRACE if c0 else 0)
     spec = solhip.spec_default(solhip.PROG_PATHTRACE)
     if c0:
         spec.enable_sky, spec.spp_per_frame, spec.max_depth, spec.nee, spec.rr_start = 0, 16, 4, 1, 0
@@ -325,7 +327,8 @@ def bench_server(a):
     from solhip import scenes, serve
     ctx = solhip.Context(0)
     sds = [scenes.cornell_box(), scenes.heightfield(1024), scenes.instanced(10000)]
-    gss = [solhip.GpuScene(ctx, sd) for sd in sds]
+    # the served scenes are static: binned SAH builds
+    gss = [solhip.GpuScene(ctx, sd, build_flags=solhip.BUILD_PREFER_FAST_TRACE) for sd in sds]
     ctx.sync()
     warm = serve.draw_requests(4, 1000.0, seed=1, resolutions=[(256, 256)], spp_max=4)
     streams = 0 if a.streams is None else a.streams

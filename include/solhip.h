@@ -232,9 +232,15 @@ int32_t solhip_buffer_fill(solhip_ctx ctx, solhip_stream s, solhip_buffer b, uin
 int32_t solhip_buffer_address(solhip_ctx ctx, solhip_buffer b, uint64_t *addr);
 
 /* ---- acceleration structures ------------------------------------------- */
-/* BLAS::new (src/ray/acceleration.rs:136-239): GPU LBVH build, enqueued on s.
+/* BLAS::new (src/ray/acceleration.rs:136-239): GPU BVH build, enqueued on s.
  * Exactly one geometry per BLAS (SceneDescription::from_meshes builds one
- * BLAS per primitive, src/ray/mod.rs:125-132). */
+ * BLAS per primitive, src/ray/mod.rs:125-132).  flags: 0 builds over Morton
+ * order (fast build, for geometry rebuilt every frame);
+ * SOLHIP_BUILD_PREFER_FAST_TRACE builds top down by binned SAH (slower build,
+ * fewer node tests per ray), the flag BLAS::new passes to Vulkan
+ * (VK_BUILD_ACCELERATION_STRUCTURE_PREFER_FAST_TRACE_BIT_KHR,
+ * src/ray/acceleration.rs:213).  solhip_blas_rebuild keeps the flags. */
+#define SOLHIP_BUILD_PREFER_FAST_TRACE 0x1u
 int32_t solhip_blas_build(solhip_ctx ctx, solhip_stream s, const solhip_geometry *g, uint32_t n,
                           uint32_t flags, solhip_blas *out);
 /* rebuild in place from the same (or updated) vertex data */

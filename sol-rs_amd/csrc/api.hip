@@ -627,13 +627,13 @@ int32_t solhip_buffer_address(solhip_ctx ctx, solhip_buffer bh, uint64_t *addr)
 int32_t solhip_blas_build(solhip_ctx ctx, solhip_stream st, const solhip_geometry *g, uint32_t n, uint32_t flags,
                           solhip_blas *out)
 {
-    (void)flags;
     Context *c = ctx_of(ctx);
     if (!c) return bad_handle("context");
     bool ok;
     hipStream_t s = stream_of(c, st, ok);
     if (!ok) return bad_handle("stream");
     if (!g || !out) return bad_arg("geometry / out");
+    if (flags & ~(uint32_t)SOLHIP_BUILD_PREFER_FAST_TRACE) return bad_arg("unknown build flags");
     if (n != 1) {
         set_error("exactly one geometry per BLAS is supported (SceneDescription builds one BLAS per primitive)");
         return SOLHIP_ERROR_UNSUPPORTED;
@@ -643,7 +643,8 @@ int32_t solhip_blas_build(solhip_ctx ctx, solhip_stream st, const solhip_geometr
     Blas *b = new Blas();
     b->kind = Kind::Blas;
     b->ctx = ctx;
-    int rc = build_blas(s, *g, b->scratch, b->bvh);
+    b->build_flags = flags;
+    int rc = build_blas(s, *g, flags, b->scratch, b->bvh);
     if (rc) {
         b->bvh.release();
         b->scratch.release();
@@ -664,7 +665,7 @@ int32_t solhip_blas_rebuild(solhip_ctx ctx, solhip_stream st, solhip_blas bh, co
     Blas *b = get<Blas>(bh, Kind::Blas);
     if (!b) return bad_handle("BLAS");
     if (!g || n != 1) return bad_arg("exactly one geometry");
-    return build_blas(s, *g, b->scratch, b->bvh);
+    return build_blas(s, *g, b->build_flags, b->scratch, b->bvh);
 }
 
 int32_t solhip_blas_destroy(solhip_ctx ctx, solhip_blas bh)

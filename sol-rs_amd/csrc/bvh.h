@@ -54,7 +54,8 @@ struct DeviceBvh {
     }
 };
 
-int build_blas(hipStream_t s, const solhip_geometry &g, DevScratch &sc, DeviceBvh &out);
+// flags: SOLHIP_BUILD_* (SOLHIP_BUILD_PREFER_FAST_TRACE: binned SAH, else Morton order)
+int build_blas(hipStream_t s, const solhip_geometry &g, uint32_t flags, DevScratch &sc, DeviceBvh &out);
 int build_tlas(hipStream_t s, const solhip_instance *inst_dev, const BlasRef *refs_dev, uint32_t n, DevScratch &sc,
                DeviceBvh &out);
 

@@ -373,6 +373,264 @@ __global__ void __launch_bounds__(kBlock) k_refit_cross(int n, const uint32_t *l
     }
 }
 
+// ---- binned SAH binary tree (fast-trace BLAS builds) ---------------------
+// Top down, one launch per tree level, one workgroup per node of the level
+// (a task: first position of the node's range in the primitive order, the
+// binary node).  The workgroup finds the node's centroid range and box, bins
+// its primitives (sah_choose), and partitions the range stably into the
+// other order buffer.  A side with one primitive is a leaf, placed at its
+// final position (leaf node n-1+position); a larger side gets the next
+// internal node index and becomes a task of the next level.  Node indices
+// follow the order in which workgroups finish, but the tree's shape does not
+// depend on it: it equals the host replay's.
+struct SahArgs {
+    const float4 *boxes;   // 2 per primitive
+    const uint32_t *ids_in;
+    uint32_t *ids_out;
+    uint32_t *vals;        // primitive of each leaf position
+    const uint2 *tasks_in;
+    uint2 *tasks_out;
+    uint32_t *ntasks_out;
+    uint32_t *node_alloc;
+    uint32_t *left, *right, *rfirst, *ncount;
+    float4 *nbox;
+    uint32_t n, task_cap, level;
+    uint32_t *err;
+};
+
+struct SahLdsBins {
+    const uint32_t (*cnt)[kSahBins];
+    const uint32_t (*lo_)[kSahBins][3];
+    const uint32_t (*hi_)[kSahBins][3];
+    __device__ uint32_t count(int a, int b) const { return cnt[a][b]; }
+    __device__ float lo(int a, int b, int k) const { return ord2f(lo_[a][b][k]); }
+    __device__ float hi(int a, int b, int k) const { return ord2f(hi_[a][b][k]); }
+};
+
+__device__ __forceinline__ void prim_centroid(const float4 *boxes, uint32_t p, float4 &l, float4 &h, float c[3])
+{
+    l = boxes[2ull * p];
+    h = boxes[2ull * p + 1];
+    c[0] = 0.5f * l.x + 0.5f * h.x;
+    c[1] = 0.5f * l.y + 0.5f * h.y;
+    c[2] = 0.5f * l.z + 0.5f * h.z;
+}
+
+__global__ void __launch_bounds__(kBlock) k_sah_level(SahArgs A)
+{
+    __shared__ uint32_t s_cnt[3][kSahBins];
+    __shared__ uint32_t s_lo[3][kSahBins][3], s_hi[3][kSahBins][3];
+    __shared__ uint32_t s_red[kBlock / 64][12];
+    __shared__ uint32_t s_bounds[12];
+    __shared__ int s_split[3];
+    __shared__ uint32_t s_wave[kBlock / 64];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint2 t = A.tasks_in[blockIdx.x];
+    const uint32_t lo = t.x, node = t.y, n = A.n;
+    // block-uniform checks: every thread of the block takes the same branch
+    if (node >= n - 1u) {
+        if (tid == 0) atomicAdd(A.err, 1u);
+        return;
+    }
+    const uint32_t cnt = A.ncount[node];
+    if (cnt < 2u || cnt > n || lo > n - cnt) {
+        if (tid == 0) atomicAdd(A.err, 1u);
+        return;
+    }
+    const uint32_t hi = lo + cnt;
+    // pass 1: centroid range and node box (ordered-integer min / max)
+    uint32_t r[12];
+    for (int k = 0; k < 3; ++k) {
+        r[k] = 0xFFFFFFFFu;      // centroid min
+        r[3 + k] = 0u;           // centroid max
+        r[6 + k] = 0xFFFFFFFFu;  // box min
+        r[9 + k] = 0u;           // box max
+    }
+    for (uint32_t i = lo + tid; i < hi; i += kBlock) {
+        const uint32_t p = min(A.ids_in[i], n - 1u);
+        float4 l, h;
+        float c[3];
+        prim_centroid(A.boxes, p, l, h, c);
+        const float lv[3] = {l.x, l.y, l.z}, hv[3] = {h.x, h.y, h.z};
+        for (int k = 0; k < 3; ++k) {
+            r[k] = min(r[k], f2ord(c[k]));
+            r[3 + k] = max(r[3 + k], f2ord(c[k]));
+            r[6 + k] = min(r[6 + k], f2ord(lv[k]));
+            r[9 + k] = max(r[9 + k], f2ord(hv[k]));
+        }
+    }
+    for (int k = 0; k < 12; ++k) {
+        const bool is_min = (k % 6) < 3;
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint32_t o = (uint32_t)__shfl_xor((int)r[k], off);
+            r[k] = is_min ? min(r[k], o) : max(r[k], o);
+        }
+    }
+    if (lane == 0)
+        for (int k = 0; k < 12; ++k) s_red[wave][k] = r[k];
+    // bins start empty
+    for (uint32_t j = tid; j < 3u * kSahBins; j += kBlock) {
+        (&s_cnt[0][0])[j] = 0u;
+        for (int k = 0; k < 3; ++k) {
+            (&s_lo[0][0][0])[3 * j + k] = f2ord(INFINITY);
+            (&s_hi[0][0][0])[3 * j + k] = f2ord(-INFINITY);
+        }
+    }
+    __syncthreads();
+    if (tid < 12) {
+        const bool is_min = (tid % 6) < 3;
+        uint32_t v = s_red[0][tid];
+        for (int w = 1; w < kBlock / 64; ++w) v = is_min ? min(v, s_red[w][tid]) : max(v, s_red[w][tid]);
+        s_bounds[tid] = v;
+    }
+    __syncthreads();
+    float cmin[3], scale[3];
+    for (int k = 0; k < 3; ++k) {
+        cmin[k] = ord2f(s_bounds[k]);
+        scale[k] = sah_scale(ord2f(s_bounds[3 + k]) - cmin[k]);
+    }
+    if (tid == 0) {
+        A.nbox[2ull * node] = make_float4(ord2f(s_bounds[6]), ord2f(s_bounds[7]), ord2f(s_bounds[8]), 0.0f);
+        A.nbox[2ull * node + 1] = make_float4(ord2f(s_bounds[9]), ord2f(s_bounds[10]), ord2f(s_bounds[11]), 0.0f);
+    }
+    // pass 2: bins.  Neighbouring primitives mostly share a bin, so a wave
+    // whose valid lanes all fall in one bin reduces them with shuffles and
+    // makes one update; otherwise each lane updates its bin.
+    for (uint32_t base = lo; base < hi; base += kBlock) {
+        const uint32_t i = base + tid;
+        const bool valid = i < hi;
+        float4 l = make_float4(INFINITY, INFINITY, INFINITY, 0.0f), h = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+        float c[3] = {0.0f, 0.0f, 0.0f};
+        if (valid) prim_centroid(A.boxes, min(A.ids_in[i], n - 1u), l, h, c);
+        const uint64_t vmask = __ballot(valid);
+        if (vmask == 0) continue;   // wave-uniform
+        const float lv[3] = {l.x, l.y, l.z}, hv[3] = {h.x, h.y, h.z};
+        for (int a = 0; a < 3; ++a) {
+            if (!(scale[a] > 0.0f)) continue;   // block-uniform
+            const int b = sah_bin(c[a], cmin[a], scale[a]);
+            const int b0 = __shfl(b, 0);         // lane 0 is valid when any lane is
+            if (__ballot(valid && b != b0) == 0) {
+                uint32_t v[6];
+                for (int k = 0; k < 3; ++k) {
+                    v[k] = f2ord(lv[k]);
+                    v[3 + k] = f2ord(hv[k]);
+                }
+                for (int off = 32; off > 0; off >>= 1)
+                    for (int k = 0; k < 6; ++k) {
+                        const uint32_t o = (uint32_t)__shfl_xor((int)v[k], off);
+                        v[k] = k < 3 ? min(v[k], o) : max(v[k], o);
+                    }
+                if (lane == 0) {
+                    atomicAdd(&s_cnt[a][b0], (uint32_t)__popcll(vmask));
+                    for (int k = 0; k < 3; ++k) {
+                        atomicMin(&s_lo[a][b0][k], v[k]);
+                        atomicMax(&s_hi[a][b0][k], v[3 + k]);
+                    }
+                }
+            } else if (valid) {
+                atomicAdd(&s_cnt[a][b], 1u);
+                for (int k = 0; k < 3; ++k) {
+                    atomicMin(&s_lo[a][b][k], f2ord(lv[k]));
+                    atomicMax(&s_hi[a][b][k], f2ord(hv[k]));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const SahSplit sp = sah_choose(SahLdsBins{s_cnt, s_lo, s_hi}, scale, cnt, A.level);
+        s_split[0] = sp.axis;
+        s_split[1] = sp.bin;
+        s_split[2] = (int)sp.nleft;
+    }
+    __syncthreads();
+    const int axis = s_split[0], sbin = s_split[1];
+    const uint32_t nleft = (uint32_t)s_split[2], nright = cnt - nleft;
+    // pass 3: stable partition into ids_out, chunk by chunk
+    uint32_t lbase = 0, rbase = 0;
+    for (uint32_t base = lo; base < hi; base += kBlock) {
+        const uint32_t i = base + tid;
+        const bool valid = i < hi;
+        uint32_t p = 0;
+        bool left_side = false;
+        float4 l, h;
+        if (valid) {
+            p = min(A.ids_in[i], n - 1u);
+            float c[3];
+            prim_centroid(A.boxes, p, l, h, c);
+            left_side = axis >= 0 ? sah_bin(c[axis], cmin[axis], scale[axis]) <= sbin : (i - lo) < nleft;
+        }
+        const uint64_t m = __ballot(left_side);
+        if (lane == 0) s_wave[wave] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t woff = 0, total = 0;
+        for (uint32_t w = 0; w < kBlock / 64; ++w) {
+            woff += w < wave ? s_wave[w] : 0u;
+            total += s_wave[w];
+        }
+        const uint32_t lrank = woff + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (valid) {
+            const uint32_t rrank = (i - base) - lrank;
+            const uint32_t pos = left_side ? lo + lbase + lrank : lo + nleft + rbase + rrank;
+            if (pos < hi && (left_side ? lbase + lrank < nleft : rbase + rrank < nright)) {
+                A.ids_out[pos] = p;
+                if ((left_side ? nleft : nright) == 1u) {
+                    // a one-primitive side: leaf n-1+pos at its final position
+                    A.vals[pos] = p;
+                    A.nbox[2ull * (n - 1u + pos)] = l;
+                    A.nbox[2ull * (n - 1u + pos) + 1] = h;
+                    A.ncount[n - 1u + pos] = 1u;
+                }
+            } else {
+                atomicAdd(A.err, 1u);
+            }
+        }
+        lbase += total;
+        rbase += min((uint32_t)kBlock, hi - base) - total;
+        __syncthreads();   // s_wave is rewritten by the next chunk
+    }
+    if (tid == 0) {
+        if (lbase != nleft || nleft == 0u || nleft >= cnt) atomicAdd(A.err, 1u);
+        for (int side = 0; side < 2; ++side) {
+            const uint32_t start = side ? lo + nleft : lo, m = side ? nright : nleft;
+            uint32_t child;
+            if (m == 1u) {
+                child = n - 1u + start;
+            } else {
+                child = atomicAdd(A.node_alloc, 1u);
+                const uint32_t slot = atomicAdd(A.ntasks_out, 1u);
+                if (child >= n - 1u || slot >= A.task_cap || m == 0u) {
+                    atomicAdd(A.err, 1u);
+                    child = 0u;
+                } else {
+                    A.rfirst[child] = start;
+                    A.ncount[child] = m;
+                    A.tasks_out[slot] = make_uint2(start, child);
+                }
+            }
+            if (side) A.right[node] = child;
+            else A.left[node] = child;
+        }
+    }
+}
+
+__global__ void k_sah_init(uint32_t n, uint2 *task0, uint32_t *ncount, uint32_t *rfirst, uint32_t *counters)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        task0[0] = make_uint2(0u, 0u);
+        ncount[0] = n;
+        rfirst[0] = 0u;
+        counters[6] = 0u;   // tasks of the next level
+        counters[7] = 1u;   // internal nodes allocated (the root)
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_iota(uint32_t *v, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) v[i] = i;
+}
+
 // ---- collapse to 8-wide quantised nodes ---------------------------------
 struct DevAlloc {
     uint32_t *counters;
@@ -506,53 +764,94 @@ void DevBuf::release()
     bytes = 0;
 }
 
-// Generic build over primitive boxes already in scratch.boxes (2 float4 per
-// primitive) with centroid bounds in scratch.bounds.
-static int build_common(hipStream_t s, uint32_t n, uint32_t leaf_max, DevScratch &sc, DeviceBvh &out)
+static int build_binary_lbvh(hipStream_t s, uint32_t n, DevScratch &sc, const uint32_t *&vals_out);
+
+// Binary tree by binned SAH (k_sah_level), one launch and one read-back of
+// the next level's task count per level.  Order buffers: vals[0] / vals[1];
+// leaf order: keys[0]; tasks: flags / frontier (at most n / 2 per level).
+static int build_binary_sah(hipStream_t s, uint32_t n, DevScratch &sc, const uint32_t *&vals_out)
 {
-    const uint32_t g = cdiv(n, kBlock);
+    uint32_t *counters = (uint32_t *)sc.counters.ptr;
+    uint32_t *pinned = sc.pinned;
+    uint32_t *ids[2] = {(uint32_t *)sc.vals[0].ptr, (uint32_t *)sc.vals[1].ptr};
+    uint32_t *vals = (uint32_t *)sc.keys[0].ptr;
+    uint2 *tasks[2] = {(uint2 *)sc.flags.ptr, (uint2 *)sc.frontier.ptr};
+    uint32_t *ncount = (uint32_t *)sc.ncount.ptr, *rfirst = (uint32_t *)sc.rfirst.ptr;
+    const uint32_t cap = n / 2;
+    hipLaunchKernelGGL(k_iota, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, ids[0], n);
+    hipLaunchKernelGGL(k_sah_init, dim3(1), dim3(64), 0, s, n, tasks[0], ncount, rfirst, counters);
+    uint32_t ntasks = 1, level = 0;
+    int cur = 0;
+    while (ntasks) {
+        SahArgs a;
+        a.boxes = (const float4 *)sc.boxes.ptr;
+        a.ids_in = ids[cur];
+        a.ids_out = ids[cur ^ 1];
+        a.vals = vals;
+        a.tasks_in = tasks[cur];
+        a.tasks_out = tasks[cur ^ 1];
+        a.ntasks_out = counters + 6;
+        a.node_alloc = counters + 7;
+        a.left = (uint32_t *)sc.left.ptr;
+        a.right = (uint32_t *)sc.right.ptr;
+        a.rfirst = rfirst;
+        a.ncount = ncount;
+        a.nbox = (float4 *)sc.nbox.ptr;
+        a.n = n;
+        a.task_cap = cap;
+        a.level = level;
+        a.err = counters + 3;
+        hipLaunchKernelGGL(k_sah_level, dim3(ntasks), dim3(kBlock), 0, s, a);
+        SOLHIP_HIP_CHECK(hipMemcpyAsync(pinned, counters, 32, hipMemcpyDeviceToHost, s));
+        SOLHIP_HIP_CHECK(hipStreamSynchronize(s));
+        if (pinned[3]) {
+            set_error("SAH BVH build range check failed at level " + std::to_string(level));
+            return SOLHIP_ERROR_DEVICE;
+        }
+        ntasks = pinned[6];
+        ++level;
+        if (ntasks > cap || level > kSahMedianLevel + 40) {
+            set_error("SAH BVH build did not converge");
+            return SOLHIP_ERROR_DEVICE;
+        }
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, counters + 6, 1u, 0u);
+        cur ^= 1;
+    }
+    if (pinned[7] != n - 1) {
+        set_error("SAH BVH build made " + std::to_string(pinned[7]) + " internal nodes for " + std::to_string(n) +
+                  " primitives");
+        return SOLHIP_ERROR_DEVICE;
+    }
+    vals_out = vals;
+    return 0;
+}
+
+// Generic build over primitive boxes already in scratch.boxes (2 float4 per
+// primitive) with centroid bounds in scratch.bounds: a binary tree (Morton
+// order and radix tree, or binned SAH when `sah`), then the 8-wide collapse.
+static int build_common(hipStream_t s, uint32_t n, uint32_t leaf_max, bool sah, DevScratch &sc, DeviceBvh &out)
+{
     // counters: [0] wide nodes allocated, [1] leaf records allocated,
-    // [2] collapse range errors, [3] sort / refit range errors, [4] invalid nodes
+    // [2] collapse range errors, [3] sort / refit / SAH range errors, [4] invalid
+    // nodes, [5] refit frontier size, [6] SAH tasks of the next level, [7] SAH
+    // internal nodes allocated
     uint32_t *counters = (uint32_t *)sc.counters.ptr;
     uint32_t *pinned = sc.pinned;
     pinned[8] = 1u;
     pinned[9] = pinned[10] = pinned[11] = pinned[12] = pinned[13] = 0u;
     SOLHIP_HIP_CHECK(hipMemcpyAsync(counters, pinned + 8, 24, hipMemcpyHostToDevice, s));
-    // Morton codes
-    hipLaunchKernelGGL(k_morton, dim3(g), dim3(kBlock), 0, s, (const float4 *)sc.boxes.ptr, n,
-                       (const uint32_t *)sc.bounds.ptr, (uint32_t *)sc.keys[0].ptr, (uint32_t *)sc.vals[0].ptr);
-    // radix sort, 4 passes of 8 bits (codes have 30 bits)
-    const uint32_t ntiles = cdiv(n, kSortTile);
-    int cur = 0;
-    for (int pass = 0; pass < 4; ++pass) {
-        const int shift = pass * 8;
-        hipLaunchKernelGGL(k_hist, dim3(ntiles), dim3(kBlock), 0, s, (const uint32_t *)sc.keys[cur].ptr, n, shift,
-                           (uint32_t *)sc.hist.ptr, ntiles);
-        int rc = exclusive_scan(s, (uint32_t *)sc.hist.ptr, 256u * ntiles, sc, 0);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_scatter, dim3(ntiles), dim3(kBlock), 0, s, (const uint32_t *)sc.keys[cur].ptr,
-                           (const uint32_t *)sc.vals[cur].ptr, n, shift, (const uint32_t *)sc.hist.ptr, ntiles,
-                           (uint32_t *)sc.keys[cur ^ 1].ptr, (uint32_t *)sc.vals[cur ^ 1].ptr, counters + 3);
-        cur ^= 1;
-    }
-    const uint32_t *keys = (const uint32_t *)sc.keys[cur].ptr;
-    const uint32_t *vals = (const uint32_t *)sc.vals[cur].ptr;
     uint32_t *left = (uint32_t *)sc.left.ptr, *right = (uint32_t *)sc.right.ptr;
-    uint32_t *parent = (uint32_t *)sc.parent.ptr, *rfirst = (uint32_t *)sc.rfirst.ptr;
-    uint32_t *flags = (uint32_t *)sc.flags.ptr;
+    uint32_t *rfirst = (uint32_t *)sc.rfirst.ptr;
     float4 *nbox = (float4 *)sc.nbox.ptr;
     uint32_t *ncount = (uint32_t *)sc.ncount.ptr;
-    if (n > 1) {
-        hipLaunchKernelGGL(k_karras, dim3(cdiv(n - 1, kBlock)), dim3(kBlock), 0, s, keys, (int)n, left, right, parent,
-                           rfirst, ncount);
-        hipLaunchKernelGGL(k_fill_u32_n, dim3(cdiv(n - 1, kBlock)), dim3(kBlock), 0, s, flags, n - 1, 0u);
+    const uint32_t *vals = nullptr;
+    if (sah && n > 1) {
+        int rc = build_binary_sah(s, n, sc, vals);
+        if (rc) return rc;
+    } else {
+        int rc = build_binary_lbvh(s, n, sc, vals);
+        if (rc) return rc;
     }
-    uint32_t *frontier = (uint32_t *)sc.frontier.ptr;
-    hipLaunchKernelGGL(k_refit_local, dim3(cdiv(n, kRefitSpan)), dim3(kBlock), 0, s, (const float4 *)sc.boxes.ptr,
-                       vals, (int)n, left, right, parent, rfirst, ncount, nbox, frontier, counters + 5, counters + 3);
-    if (n > 1)
-        hipLaunchKernelGGL(k_refit_cross, dim3(std::min<uint32_t>(cdiv(n, kBlock), 2048)), dim3(kBlock), 0, s, (int)n,
-                           left, right, parent, flags, nbox, frontier, counters + 5, counters + 3);
     // binary root: internal node 0, or leaf 0 (node index n-1 = 0) when n == 1
     const uint32_t root = 0u;
     hipLaunchKernelGGL(k_root_box, dim3(1), dim3(64), 0, s, (const float4 *)nbox, root, (float *)out.root_box.ptr);
@@ -612,6 +911,51 @@ static int build_common(hipStream_t s, uint32_t n, uint32_t leaf_max, DevScratch
     return 0;
 }
 
+// Binary tree over Morton order: codes, LSD radix sort, Karras radix tree,
+// two-phase refit.  Leaf order: the sorted values.
+static int build_binary_lbvh(hipStream_t s, uint32_t n, DevScratch &sc, const uint32_t *&vals_out)
+{
+    const uint32_t g = cdiv(n, kBlock);
+    uint32_t *counters = (uint32_t *)sc.counters.ptr;
+    // Morton codes
+    hipLaunchKernelGGL(k_morton, dim3(g), dim3(kBlock), 0, s, (const float4 *)sc.boxes.ptr, n,
+                       (const uint32_t *)sc.bounds.ptr, (uint32_t *)sc.keys[0].ptr, (uint32_t *)sc.vals[0].ptr);
+    // radix sort, 4 passes of 8 bits (codes have 30 bits)
+    const uint32_t ntiles = cdiv(n, kSortTile);
+    int cur = 0;
+    for (int pass = 0; pass < 4; ++pass) {
+        const int shift = pass * 8;
+        hipLaunchKernelGGL(k_hist, dim3(ntiles), dim3(kBlock), 0, s, (const uint32_t *)sc.keys[cur].ptr, n, shift,
+                           (uint32_t *)sc.hist.ptr, ntiles);
+        int rc = exclusive_scan(s, (uint32_t *)sc.hist.ptr, 256u * ntiles, sc, 0);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_scatter, dim3(ntiles), dim3(kBlock), 0, s, (const uint32_t *)sc.keys[cur].ptr,
+                           (const uint32_t *)sc.vals[cur].ptr, n, shift, (const uint32_t *)sc.hist.ptr, ntiles,
+                           (uint32_t *)sc.keys[cur ^ 1].ptr, (uint32_t *)sc.vals[cur ^ 1].ptr, counters + 3);
+        cur ^= 1;
+    }
+    const uint32_t *keys = (const uint32_t *)sc.keys[cur].ptr;
+    const uint32_t *vals = (const uint32_t *)sc.vals[cur].ptr;
+    uint32_t *left = (uint32_t *)sc.left.ptr, *right = (uint32_t *)sc.right.ptr;
+    uint32_t *parent = (uint32_t *)sc.parent.ptr, *rfirst = (uint32_t *)sc.rfirst.ptr;
+    uint32_t *flags = (uint32_t *)sc.flags.ptr;
+    float4 *nbox = (float4 *)sc.nbox.ptr;
+    uint32_t *ncount = (uint32_t *)sc.ncount.ptr;
+    if (n > 1) {
+        hipLaunchKernelGGL(k_karras, dim3(cdiv(n - 1, kBlock)), dim3(kBlock), 0, s, keys, (int)n, left, right, parent,
+                           rfirst, ncount);
+        hipLaunchKernelGGL(k_fill_u32_n, dim3(cdiv(n - 1, kBlock)), dim3(kBlock), 0, s, flags, n - 1, 0u);
+    }
+    uint32_t *frontier = (uint32_t *)sc.frontier.ptr;
+    hipLaunchKernelGGL(k_refit_local, dim3(cdiv(n, kRefitSpan)), dim3(kBlock), 0, s, (const float4 *)sc.boxes.ptr,
+                       vals, (int)n, left, right, parent, rfirst, ncount, nbox, frontier, counters + 5, counters + 3);
+    if (n > 1)
+        hipLaunchKernelGGL(k_refit_cross, dim3(std::min<uint32_t>(cdiv(n, kBlock), 2048)), dim3(kBlock), 0, s, (int)n,
+                           left, right, parent, flags, nbox, frontier, counters + 5, counters + 3);
+    vals_out = vals;
+    return 0;
+}
+
 int DevScratch::ensure(uint32_t n)
 {
     const uint64_t N = std::max<uint32_t>(n, 1);
@@ -658,7 +1002,7 @@ void DevScratch::release()
 
 static const uint32_t kOrdInit[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
 
-int build_blas(hipStream_t s, const solhip_geometry &g, DevScratch &sc, DeviceBvh &out)
+int build_blas(hipStream_t s, const solhip_geometry &g, uint32_t flags, DevScratch &sc, DeviceBvh &out)
 {
     if (g.index_count % 3 != 0) {
         set_error("index_count must be a multiple of 3");
@@ -685,7 +1029,7 @@ int build_blas(hipStream_t s, const solhip_geometry &g, DevScratch &sc, DeviceBv
     const uint32_t *idx = reinterpret_cast<const uint32_t *>(g.index_addr + g.index_byte_offset);
     hipLaunchKernelGGL(k_tri_boxes, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, vbase, g.vertex_stride, g.first_vertex,
                        g.vertex_count, idx, n, (float4 *)sc.boxes.ptr, (uint32_t *)sc.bounds.ptr);
-    rc = build_common(s, n, kLeafMaxTri, sc, out);
+    rc = build_common(s, n, kLeafMaxTri, (flags & SOLHIP_BUILD_PREFER_FAST_TRACE) != 0, sc, out);
     if (rc) return rc;
     hipLaunchKernelGGL(k_write_tris, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, (const uint32_t *)sc.leaf_ids.ptr, n,
                        vbase, g.vertex_stride, g.first_vertex, g.vertex_count, idx, (TriRec *)out.leaves.ptr);
@@ -715,7 +1059,7 @@ int build_tlas(hipStream_t s, const solhip_instance *inst_dev, const BlasRef *re
     SOLHIP_HIP_CHECK(hipMemcpyAsync(sc.bounds.ptr, kOrdInit, 24, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_inst_boxes, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, inst_dev, refs_dev, n,
                        (float4 *)sc.boxes.ptr, (uint32_t *)sc.bounds.ptr);
-    rc = build_common(s, n, kLeafMaxInst, sc, out);
+    rc = build_common(s, n, kLeafMaxInst, false, sc, out);
     if (rc) return rc;
     hipLaunchKernelGGL(k_write_insts, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, (const uint32_t *)sc.leaf_ids.ptr, n,
                        inst_dev, refs_dev, (InstRec *)out.leaves.ptr, (uint32_t *)out.meta.ptr);

@@ -101,6 +101,87 @@ SOLHIP_HD float half_area(float4 lo, float4 hi)
     return dx * dy + dy * dz + dz * dx;
 }
 
+// ---- binned SAH split (the fast-trace BLAS build) ------------------------
+// A node's primitives are binned by centroid into kSahBins equal bins per
+// axis over the node's centroid range; the split between two bins with the
+// least cost A(left) * N(left) + A(right) * N(right) (A = half area of the
+// union of the side's primitive boxes) is taken; ties keep the first found
+// (axis x, y, z, bin ascending).  Without a split that leaves both sides
+// non-empty (all centroids in one bin, or a zero range), and at every level
+// from kSahMedianLevel on (a depth bound), the range is halved in its current
+// order instead.
+constexpr int kSahBins = 32;
+constexpr uint32_t kSahMedianLevel = 48;
+
+SOLHIP_HD float half_area3(const float l[3], const float h[3])
+{
+    const float dx = h[0] - l[0], dy = h[1] - l[1], dz = h[2] - l[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// scale of an axis with centroid extent ext (0: the axis cannot be split)
+SOLHIP_HD float sah_scale(float ext) { return ext > 0.0f && ext < INFINITY ? (float)kSahBins / ext : 0.0f; }
+
+SOLHIP_HD int sah_bin(float c, float cmin, float scale)
+{
+    const float q = (c - cmin) * scale;
+    if (!(q > 0.0f)) return 0;
+    if (q >= (float)(kSahBins - 1)) return kSahBins - 1;
+    return (int)q;
+}
+
+struct SahSplit {
+    int axis;        // -1: halve the range in its current order
+    int bin;         // left side = bins 0..bin of `axis`
+    uint32_t nleft;
+};
+
+// Bins: count(a, b), lo(a, b, k), hi(a, b, k) (empty bins: +inf / -inf).
+template <class Bins>
+SOLHIP_HD SahSplit sah_choose(const Bins &B, const float scale[3], uint32_t count, uint32_t level)
+{
+    SahSplit best{-1, 0, count / 2};
+    if (level >= kSahMedianLevel) return best;
+    float best_cost = INFINITY;
+    for (int a = 0; a < 3; ++a) {
+        if (!(scale[a] > 0.0f)) continue;
+        float ra[kSahBins];
+        uint32_t rc[kSahBins];
+        float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+        uint32_t c = 0;
+        for (int b = kSahBins - 1; b > 0; --b) {
+            c += B.count(a, b);
+            for (int k = 0; k < 3; ++k) {
+                l[k] = fminf(l[k], B.lo(a, b, k));
+                h[k] = fmaxf(h[k], B.hi(a, b, k));
+            }
+            ra[b] = c ? half_area3(l, h) : 0.0f;
+            rc[b] = c;
+        }
+        for (int k = 0; k < 3; ++k) {
+            l[k] = INFINITY;
+            h[k] = -INFINITY;
+        }
+        c = 0;
+        for (int b = 0; b < kSahBins - 1; ++b) {
+            c += B.count(a, b);
+            for (int k = 0; k < 3; ++k) {
+                l[k] = fminf(l[k], B.lo(a, b, k));
+                h[k] = fmaxf(h[k], B.hi(a, b, k));
+            }
+            if (c == 0 || rc[b + 1] == 0) continue;
+            const float cost = half_area3(l, h) * (float)c + ra[b + 1] * (float)rc[b + 1];
+            if (cost < best_cost) {
+                best_cost = cost;
+                best.axis = a;
+                best.bin = b;
+                best.nleft = c;
+            }
+        }
+    }
+    return best;
+}
+
 // smallest power of two 2^e (e >= -126) with 2^e >= x, as biased exponent
 SOLHIP_HD uint32_t pow2_ceil_exp(float x)
 {

@@ -97,6 +97,7 @@ struct Buffer : Obj {
 struct Blas : Obj {
     DeviceBvh bvh;
     DevScratch scratch;
+    uint32_t build_flags = 0;   // SOLHIP_BUILD_*, kept for rebuilds
 };
 
 struct Tlas : Obj {

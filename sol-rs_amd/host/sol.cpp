@@ -264,7 +264,10 @@ BLAS::BLAS(std::shared_ptr<Context> ctx, CommandBuffer &cmd, const std::vector<G
         }
         g.push_back(s);
     }
-    check(solhip_blas_build(ctx_->handle(), cmd.stream, g.data(), (uint32_t)g.size(), 0, &blas_), "BLAS::new");
+    // build flags PREFER_FAST_TRACE (src/ray/acceleration.rs:213)
+    check(solhip_blas_build(ctx_->handle(), cmd.stream, g.data(), (uint32_t)g.size(), SOLHIP_BUILD_PREFER_FAST_TRACE,
+                            &blas_),
+          "BLAS::new");
 }
 BLAS::~BLAS()
 {

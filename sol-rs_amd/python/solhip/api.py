@@ -17,6 +17,7 @@ HEADER_PATH = os.path.normpath(os.path.join(_HERE, "..", "..", "..", "include", 
 PROG_AO_GLSL, PROG_PATHTRACE_GLSL, PROG_DEBUG, PROG_AO, PROG_PATHTRACE = 1, 2, 3, 4, 5
 TRACE_TERMINATE_ON_FIRST_HIT = 0x4
 TRACE_COUNT = 0x100
+BUILD_PREFER_FAST_TRACE = 0x1
 
 ERRORS = {0: "OK", -1: "INVALID_ARGUMENT", -2: "INVALID_HANDLE", -3: "OUT_OF_MEMORY", -4: "NO_DEVICE",
           -5: "DEVICE", -6: "UNSUPPORTED", -7: "OVERFLOW"}
@@ -243,10 +244,11 @@ class Context:
         return h.value
 
     def blas_build(self, vbuf, stride, vertex_count, ibuf, index_count, first_vertex=0, index_byte_offset=0,
-                   stream=0):
+                   stream=0, flags=0):
+        """flags: BUILD_PREFER_FAST_TRACE (binned SAH) or 0 (Morton order)"""
         g = Geometry(vbuf.addr, stride, first_vertex, vertex_count, ibuf.addr, index_byte_offset, index_count, 1)
         h = C.c_uint64()
-        check(lib().solhip_blas_build(self.h, stream, C.byref(g), 1, 0, C.byref(h)))
+        check(lib().solhip_blas_build(self.h, stream, C.byref(g), 1, flags, C.byref(h)))
         return h.value
 
     def blas_rebuild(self, blas, vbuf, stride, vertex_count, ibuf, index_count, stream=0):
@@ -331,7 +333,7 @@ class GpuScene:
     BLASes and TLAS: what SceneDescription::from_scene does for the Vulkan
     path (src/ray/mod.rs:59-156)."""
 
-    def __init__(self, ctx, sd, index_size=4):
+    def __init__(self, ctx, sd, index_size=4, build_flags=0):
         self.ctx = ctx
         self.sd = sd
         self.bufs = []
@@ -340,7 +342,7 @@ class GpuScene:
             vb = ctx.upload(b.vertices)
             ib = ctx.upload(b.indices.astype(np.uint32))
             self.bufs += [vb, ib]
-            self.blas.append(ctx.blas_build(vb, 64, len(b.vertices), ib, len(b.indices)))
+            self.blas.append(ctx.blas_build(vb, 64, len(b.vertices), ib, len(b.indices), flags=build_flags))
         descs = sd.instance_descs.copy()
         descs["blas"] = [self.blas[int(k)] for k in sd.instance_descs["blas"]] if len(descs) else []
         self.descs = descs

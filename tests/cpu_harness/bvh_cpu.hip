@@ -24,8 +24,98 @@ struct HostBvh {
     float root_box[6];
 };
 
+uint32_t g_build_flags = 0;   // SOLHIP_BUILD_* for BLAS builds
+
+struct HostBins {
+    uint32_t c[3][kSahBins];
+    float l[3][kSahBins][3], h[3][kSahBins][3];
+    uint32_t count(int a, int b) const { return c[a][b]; }
+    float lo(int a, int b, int k) const { return l[a][b][k]; }
+    float hi(int a, int b, int k) const { return h[a][b][k]; }
+};
+
+// binned SAH binary tree, level by level as the GPU build (k_sah_level)
+void sah_tree(const float *pbox, uint32_t n, std::vector<uint32_t> &vals, std::vector<uint32_t> &left,
+              std::vector<uint32_t> &right, std::vector<uint32_t> &rfirst, std::vector<uint32_t> &ncount)
+{
+    std::vector<uint32_t> ids(n), out(n);
+    std::iota(ids.begin(), ids.end(), 0u);
+    vals.assign(n, 0u);
+    struct Task {
+        uint32_t lo, node;
+    };
+    std::vector<Task> tasks{{0u, 0u}}, next;
+    ncount[0] = n;
+    rfirst[0] = 0;
+    uint32_t alloc = 1;
+    auto cen = [&](uint32_t p, int a) { return 0.5f * pbox[6 * p + a] + 0.5f * pbox[6 * p + 3 + a]; };
+    for (uint32_t level = 0; !tasks.empty(); ++level) {
+        next.clear();
+        for (const Task &t : tasks) {
+            const uint32_t lo = t.lo, cnt = ncount[t.node], hi = lo + cnt;
+            float cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (uint32_t i = lo; i < hi; ++i)
+                for (int a = 0; a < 3; ++a) {
+                    cmin[a] = fminf(cmin[a], cen(ids[i], a));
+                    cmax[a] = fmaxf(cmax[a], cen(ids[i], a));
+                }
+            float scale[3];
+            for (int a = 0; a < 3; ++a) scale[a] = sah_scale(cmax[a] - cmin[a]);
+            HostBins B;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < kSahBins; ++b) {
+                    B.c[a][b] = 0;
+                    for (int k = 0; k < 3; ++k) {
+                        B.l[a][b][k] = INFINITY;
+                        B.h[a][b][k] = -INFINITY;
+                    }
+                }
+            for (uint32_t i = lo; i < hi; ++i) {
+                const uint32_t p = ids[i];
+                for (int a = 0; a < 3; ++a) {
+                    if (!(scale[a] > 0.0f)) continue;
+                    const int b = sah_bin(cen(p, a), cmin[a], scale[a]);
+                    B.c[a][b] += 1;
+                    for (int k = 0; k < 3; ++k) {
+                        B.l[a][b][k] = fminf(B.l[a][b][k], pbox[6 * p + k]);
+                        B.h[a][b][k] = fmaxf(B.h[a][b][k], pbox[6 * p + 3 + k]);
+                    }
+                }
+            }
+            const SahSplit sp = sah_choose(B, scale, cnt, level);
+            uint32_t nl = 0, nr = 0;
+            for (uint32_t i = lo; i < hi; ++i) {
+                const uint32_t p = ids[i];
+                const bool l = sp.axis >= 0 ? sah_bin(cen(p, sp.axis), cmin[sp.axis], scale[sp.axis]) <= sp.bin
+                                            : (i - lo) < sp.nleft;
+                if (l)
+                    out[lo + nl++] = p;
+                else
+                    out[lo + sp.nleft + nr++] = p;
+            }
+            for (int side = 0; side < 2; ++side) {
+                const uint32_t start = side ? lo + sp.nleft : lo, m = side ? cnt - sp.nleft : sp.nleft;
+                uint32_t child;
+                if (m == 1) {
+                    child = n - 1 + start;
+                    vals[start] = out[start];
+                } else {
+                    child = alloc++;
+                    rfirst[child] = start;
+                    ncount[child] = m;
+                    next.push_back({start, child});
+                }
+                (side ? right : left)[t.node] = child;
+            }
+        }
+        for (const Task &t : next)
+            for (uint32_t i = t.lo; i < t.lo + ncount[t.node]; ++i) ids[i] = out[i];
+        tasks.swap(next);
+    }
+}
+
 // prim boxes: n x (lo3, hi3)
-int build(const float *pbox, uint32_t n, uint32_t leaf_max, HostBvh &out)
+int build(const float *pbox, uint32_t n, uint32_t leaf_max, HostBvh &out, bool sah = false)
 {
     out.nodes.clear();
     out.leaf_ids.assign(n, 0);
@@ -46,7 +136,16 @@ int build(const float *pbox, uint32_t n, uint32_t leaf_max, HostBvh &out)
     const uint32_t nb = 2 * n - 1;
     std::vector<uint32_t> left(n), right(n), parent(nb, 0xFFFFFFFFu), rfirst(n), ncount(nb, 0);
     std::vector<float4> nbox(2 * (size_t)nb);
-    for (int i = 0; i + 1 < (int)n; ++i) karras_node(skeys.data(), (int)n, i, left.data(), right.data(), parent.data(), rfirst.data());
+    if (sah && n > 1) {
+        sah_tree(pbox, n, vals, left, right, rfirst, ncount);
+        for (uint32_t i = 0; i + 1 < n; ++i) {
+            parent[left[i]] = i;
+            parent[right[i]] = i;
+        }
+    } else {
+        for (int i = 0; i + 1 < (int)n; ++i)
+            karras_node(skeys.data(), (int)n, i, left.data(), right.data(), parent.data(), rfirst.data());
+    }
     if (n > 1) parent[0] = 0xFFFFFFFFu;
     // every node except the root must have exactly one parent
     for (uint32_t k = 1; k < nb; ++k)
@@ -157,7 +256,7 @@ void *hc_scene_create(const float *const *tris, const uint32_t *ntris, uint32_t 
                 pbox[6 * i + a] = fminf(fminf(t[a], t[3 + a]), t[6 + a]);
                 pbox[6 * i + 3 + a] = fmaxf(fmaxf(t[a], t[3 + a]), t[6 + a]);
             }
-        int rc = build(pbox.data(), n, kLeafMaxTri, s->blas[b].bvh);
+        int rc = build(pbox.data(), n, kLeafMaxTri, s->blas[b].bvh, (g_build_flags & 1u) != 0);
         if (rc) {
             *err = rc;
             delete s;
@@ -201,6 +300,10 @@ void *hc_scene_create(const float *const *tris, const uint32_t *ntris, uint32_t 
 }
 
 void hc_scene_destroy(void *h) { delete static_cast<HostScene *>(h); }
+
+// BLAS build flags of the next hc_scene_create (1: binned SAH, as
+// SOLHIP_BUILD_PREFER_FAST_TRACE)
+void hc_set_build_flags(uint32_t flags) { g_build_flags = flags; }
 
 void hc_scene_info(void *h, uint32_t blas, uint32_t *nodes, uint32_t *depth)
 {

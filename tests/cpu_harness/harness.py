@@ -25,12 +25,14 @@ def lib():
         L.hc_scene_info.argtypes = [vp, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.hc_trace.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, vp]
         L.hc_trace_steps.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, vp]
+        L.hc_set_build_flags.argtypes = [C.c_uint32]
         _lib = L
     return _lib
 
 
 class HostScene:
-    def __init__(self, sd):
+    def __init__(self, sd, build_flags=0):
+        """build_flags: SOLHIP_BUILD_* of the BLAS builds (1: binned SAH)"""
         self._keep = []
         nb = len(sd.blas)
         ptrs = (C.c_void_p * max(1, nb))()
@@ -46,6 +48,7 @@ class HostScene:
         im = np.ascontiguousarray((sd.instance_descs["id_mask"] >> 24).astype(np.uint32)) if n else np.zeros(1, np.uint32)
         self._keep += [xf, ib, im, counts]
         err = C.c_int(0)
+        lib().hc_set_build_flags(build_flags)
         self.h = lib().hc_scene_create(C.cast(ptrs, C.c_void_p), counts.ctypes.data, nb, xf.ctypes.data,
                                        ib.ctypes.data, im.ctypes.data, n, C.byref(err))
         if not self.h:

@@ -28,10 +28,12 @@ def _built():
         hbuild.build()
 
 
+@pytest.mark.parametrize("flags", [0, 1], ids=["morton", "sah"])
 @pytest.mark.parametrize("name", sorted(SCENES))
-def test_replay_matches_brute_force(oracle, name):
+def test_replay_matches_brute_force(oracle, name, flags):
+    """both BLAS builds (Morton order, binned SAH) give the brute-force hits"""
     sd = SCENES[name]()
-    hs = harness.HostScene(sd)
+    hs = harness.HostScene(sd, build_flags=flags)
     sc = oracle.Scene(sd)
     rays = np.concatenate([raygen.random_rays(2000, (0, 0, 0), 2.0 if name != "instances" else 6.0, seed=11),
                            raygen.aimed_rays(sd, 1500, seed=12), raygen.axis_rays(500),
@@ -50,6 +52,16 @@ def test_replay_tree_shape_heightfield():
     hs = harness.HostScene(sd)
     nodes, depth = hs.info(0)
     assert nodes < sd.tri_count // 2 and 3 <= depth <= 16
+
+
+def test_replay_sah_build_needs_fewer_node_tests(oracle):
+    """on a terrain the binned SAH build needs fewer node and triangle tests
+    per camera ray than the Morton-order build, for the same hits"""
+    sd = scenes.heightfield(128)
+    rays = raygen.camera_rays(sd, 64, 36, oracle)
+    (gm, cm), (gs, cs) = (harness.HostScene(sd, build_flags=f).trace(rays) for f in (0, 1))
+    assert np.array_equal(gm.view(np.uint32), gs.view(np.uint32))
+    assert cs[0] < 0.95 * cm[0] and cs[1] <= cm[1]
 
 
 @pytest.mark.parametrize("name", sorted(SCENES))

@@ -38,11 +38,12 @@ AO_SCENES = {
 }
 
 
+@pytest.mark.parametrize("build", [0, solhip.BUILD_PREFER_FAST_TRACE], ids=["morton", "sah"])
 @pytest.mark.parametrize("name", sorted(AO_SCENES))
-def test_ao_matches_oracle_three_frames(ctx, oracle, name):
+def test_ao_matches_oracle_three_frames(ctx, oracle, name, build):
     f, W, H = AO_SCENES[name]
     sd = f()
-    gs = solhip.GpuScene(ctx, sd)
+    gs = solhip.GpuScene(ctx, sd, build_flags=build)
     spec = solhip.spec_default(solhip.PROG_AO)
     prog = ctx.program(solhip.PROG_AO, spec)
     img, acc, _, hits = _images(ctx, W, H, n_hits=W * H * (1 + spec.n_ao))

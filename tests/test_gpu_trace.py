@@ -12,8 +12,8 @@ from solhip.records import HIT
 pytestmark = pytest.mark.gpu
 
 
-def _check(ctx, oracle, sd, rays, brute=True, flags=0):
-    gs = solhip.GpuScene(ctx, sd)
+def _check(ctx, oracle, sd, rays, brute=True, flags=0, build=0):
+    gs = solhip.GpuScene(ctx, sd, build_flags=build)
     ctx.stats(reset=True)
     g = ctx.trace_rays(gs.tlas, rays, flags)
     st = ctx.stats()
@@ -38,18 +38,23 @@ SMALL_SCENES = {
 }
 
 
+BUILDS = pytest.mark.parametrize("build", [0, solhip.BUILD_PREFER_FAST_TRACE], ids=["morton", "sah"])
+
+
+@BUILDS
 @pytest.mark.parametrize("name", sorted(SMALL_SCENES))
-def test_trace_random_rays_match_brute_force(ctx, oracle, name):
+def test_trace_random_rays_match_brute_force(ctx, oracle, name, build):
     sd = SMALL_SCENES[name]()
     rays = raygen.random_rays(4000, (0, 0, 0), 2.0 if name != "instances" else 6.0, seed=11)
-    _check(ctx, oracle, sd, rays)
+    _check(ctx, oracle, sd, rays, build=build)
 
 
+@BUILDS
 @pytest.mark.parametrize("name", sorted(SMALL_SCENES))
-def test_trace_aimed_rays_match_brute_force(ctx, oracle, name):
+def test_trace_aimed_rays_match_brute_force(ctx, oracle, name, build):
     sd = SMALL_SCENES[name]()
     rays = raygen.aimed_rays(sd, 3000, seed=12)
-    g, r = _check(ctx, oracle, sd, rays)
+    g, r = _check(ctx, oracle, sd, rays, build=build)
     # rays aimed at vertices / shared edges of closed meshes must not leak
     if name in ("heightfield64",):
         assert (g["inst"] != 0xFFFFFFFF).mean() > 0.5
@@ -102,8 +107,27 @@ def test_bvh_info_and_counts(ctx, oracle):
     assert st.node_fetches >= len(rays) and st.inst_fetches >= 1
 
 
-def test_heightfield_large_sampled(ctx, oracle):
+@BUILDS
+def test_heightfield_large_sampled(ctx, oracle, build):
     """512^2 heightfield (524k triangles) against the oracle's own BVH"""
     sd = scenes.heightfield(512)
     rays = np.concatenate([raygen.camera_rays(sd, 64, 36, oracle), raygen.aimed_rays(sd, 4000, seed=21, dist=0.5)])
-    _check(ctx, oracle, sd, rays, brute=False)
+    _check(ctx, oracle, sd, rays, brute=False, build=build)
+
+
+@BUILDS
+@pytest.mark.parametrize("name", ["soup", "instances", "heightfield128"])
+def test_gpu_tree_equals_host_replay(ctx, oracle, name, build):
+    """the GPU build makes the tree the host replay of the same steps makes
+    (tests/cpu_harness): same hits and the same node, triangle and instance
+    fetch counts over a ray set"""
+    from cpu_harness import harness
+    sd = scenes.heightfield(128) if name == "heightfield128" else SMALL_SCENES[name]()
+    rays = np.concatenate([raygen.camera_rays(sd, 64, 36, oracle), raygen.random_rays(3000, (0, 0, 0), 3.0, seed=4)])
+    gs = solhip.GpuScene(ctx, sd, build_flags=build)
+    ctx.stats(reset=True)
+    g = ctx.trace_rays(gs.tlas, rays, solhip.TRACE_COUNT)
+    st = ctx.stats(reset=True)
+    h, c = harness.HostScene(sd, build_flags=build).trace(rays)
+    assert np.array_equal(g.view(np.uint32), h.view(np.uint32))
+    assert (st.node_fetches, st.tri_fetches, st.inst_fetches) == (int(c[0]), int(c[1]), int(c[2]))
