@@ -92,6 +92,35 @@ def test_empty_tlas_misses(ctx):
     assert (g["inst"] == 0xFFFFFFFF).all() and (g["t"] == -1.0).all()
 
 
+def test_blas_build_rejects_unknown_flags(ctx):
+    sd = scenes.single_triangle()
+    b = sd.blas[0]
+    vb = ctx.upload(b.vertices)
+    ib = ctx.upload(b.indices.astype(np.uint32))
+    with pytest.raises(solhip.SolhipError):
+        ctx.blas_build(vb, 64, len(b.vertices), ib, len(b.indices), flags=0x80)
+
+
+@BUILDS
+def test_blas_rebuild_keeps_build_flags(ctx, oracle, build):
+    """a rebuild in place uses the flags of the first build: same tree, same
+    fetch counts"""
+    sd = scenes.heightfield(64)
+    gs = solhip.GpuScene(ctx, sd, build_flags=build)
+    rays = raygen.camera_rays(sd, 48, 32, oracle)
+    counts = []
+    for rebuild in (False, True):
+        if rebuild:
+            b = sd.blas[0]
+            ctx.blas_rebuild(gs.blas[0], gs.bufs[0], 64, len(b.vertices), gs.bufs[1], len(b.indices))
+            gs.update_instances(sd.instance_descs)
+        ctx.stats(reset=True)
+        ctx.trace_rays(gs.tlas, rays, solhip.TRACE_COUNT)
+        st = ctx.stats(reset=True)
+        counts.append((st.node_fetches, st.tri_fetches))
+    assert counts[0] == counts[1]
+
+
 def test_bvh_info_and_counts(ctx, oracle):
     sd = scenes.heightfield(128)
     gs = solhip.GpuScene(ctx, sd)
