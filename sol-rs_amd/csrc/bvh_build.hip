@@ -382,19 +382,24 @@ __global__ void __launch_bounds__(kBlock) k_refit_cross(int n, const uint32_t *l
 // final position (leaf node n-1+position); a larger side gets the next
 // internal node index and becomes a task of the next level.  Node indices
 // follow the order in which workgroups finish, but the tree's shape does not
-// depend on it: it equals the host replay's.
+// depend on it: it equals the host replay's.  Nodes of at least big_min
+// primitives go to a second task list, run by 1024-thread workgroups (the
+// top levels, where a few workgroups work through millions of primitives).
+constexpr uint32_t kSahBigMin = 65536;
+
 struct SahArgs {
     const float4 *boxes;   // 2 per primitive
     const uint32_t *ids_in;
     uint32_t *ids_out;
     uint32_t *vals;        // primitive of each leaf position
-    const uint2 *tasks_in;
-    uint2 *tasks_out;
-    uint32_t *ntasks_out;
+    const uint2 *tasks_in;     // this launch's tasks (small or big list)
+    uint2 *tasks_out;          // next level, nodes of fewer than big_min primitives
+    uint2 *big_out;            // next level, nodes of big_min or more
+    uint32_t *ntasks_out, *nbig_out;
     uint32_t *node_alloc;
     uint32_t *left, *right, *rfirst, *ncount;
     float4 *nbox;
-    uint32_t n, task_cap, level;
+    uint32_t n, task_cap, big_cap, big_min, level;
     uint32_t *err;
 };
 
@@ -416,14 +421,17 @@ __device__ __forceinline__ void prim_centroid(const float4 *boxes, uint32_t p, f
     c[2] = 0.5f * l.z + 0.5f * h.z;
 }
 
-__global__ void __launch_bounds__(kBlock) k_sah_level(SahArgs A)
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_sah_level(SahArgs A)
 {
     __shared__ uint32_t s_cnt[3][kSahBins];
     __shared__ uint32_t s_lo[3][kSahBins][3], s_hi[3][kSahBins][3];
-    __shared__ uint32_t s_red[kBlock / 64][12];
+    __shared__ uint32_t s_red[BLOCK / 64][12];
     __shared__ uint32_t s_bounds[12];
+    __shared__ SahSplit s_axis_best[3];
+    __shared__ float s_axis_cost[3];
     __shared__ int s_split[3];
-    __shared__ uint32_t s_wave[kBlock / 64];
+    __shared__ uint32_t s_wave[BLOCK / 64];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint2 t = A.tasks_in[blockIdx.x];
     const uint32_t lo = t.x, node = t.y, n = A.n;
@@ -446,7 +454,7 @@ __global__ void __launch_bounds__(kBlock) k_sah_level(SahArgs A)
         r[6 + k] = 0xFFFFFFFFu;  // box min
         r[9 + k] = 0u;           // box max
     }
-    for (uint32_t i = lo + tid; i < hi; i += kBlock) {
+    for (uint32_t i = lo + tid; i < hi; i += BLOCK) {
         const uint32_t p = min(A.ids_in[i], n - 1u);
         float4 l, h;
         float c[3];
@@ -469,7 +477,7 @@ __global__ void __launch_bounds__(kBlock) k_sah_level(SahArgs A)
     if (lane == 0)
         for (int k = 0; k < 12; ++k) s_red[wave][k] = r[k];
     // bins start empty
-    for (uint32_t j = tid; j < 3u * kSahBins; j += kBlock) {
+    for (uint32_t j = tid; j < 3u * kSahBins; j += BLOCK) {
         (&s_cnt[0][0])[j] = 0u;
         for (int k = 0; k < 3; ++k) {
             (&s_lo[0][0][0])[3 * j + k] = f2ord(INFINITY);
@@ -480,7 +488,7 @@ __global__ void __launch_bounds__(kBlock) k_sah_level(SahArgs A)
     if (tid < 12) {
         const bool is_min = (tid % 6) < 3;
         uint32_t v = s_red[0][tid];
-        for (int w = 1; w < kBlock / 64; ++w) v = is_min ? min(v, s_red[w][tid]) : max(v, s_red[w][tid]);
+        for (int w = 1; w < BLOCK / 64; ++w) v = is_min ? min(v, s_red[w][tid]) : max(v, s_red[w][tid]);
         s_bounds[tid] = v;
     }
     __syncthreads();
@@ -496,7 +504,7 @@ __global__ void __launch_bounds__(kBlock) k_sah_level(SahArgs A)
     // pass 2: bins.  Neighbouring primitives mostly share a bin, so a wave
     // whose valid lanes all fall in one bin reduces them with shuffles and
     // makes one update; otherwise each lane updates its bin.
-    for (uint32_t base = lo; base < hi; base += kBlock) {
+    for (uint32_t base = lo; base < hi; base += BLOCK) {
         const uint32_t i = base + tid;
         const bool valid = i < hi;
         float4 l = make_float4(INFINITY, INFINITY, INFINITY, 0.0f), h = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
@@ -537,8 +545,24 @@ __global__ void __launch_bounds__(kBlock) k_sah_level(SahArgs A)
         }
     }
     __syncthreads();
+    // split: one thread per axis, then the first strict minimum in axis order
+    // (sah_choose's result)
+    if (tid < 3) {
+        SahSplit sp{-1, 0, cnt / 2};
+        float cost = INFINITY;
+        if (A.level < kSahMedianLevel && scale[tid] > 0.0f) sah_axis(SahLdsBins{s_cnt, s_lo, s_hi}, (int)tid, cost, sp);
+        s_axis_best[tid] = sp;
+        s_axis_cost[tid] = cost;
+    }
+    __syncthreads();
     if (tid == 0) {
-        const SahSplit sp = sah_choose(SahLdsBins{s_cnt, s_lo, s_hi}, scale, cnt, A.level);
+        SahSplit sp{-1, 0, cnt / 2};
+        float cost = INFINITY;
+        for (int a = 0; a < 3; ++a)
+            if (s_axis_cost[a] < cost) {
+                cost = s_axis_cost[a];
+                sp = s_axis_best[a];
+            }
         s_split[0] = sp.axis;
         s_split[1] = sp.bin;
         s_split[2] = (int)sp.nleft;
@@ -548,7 +572,7 @@ __global__ void __launch_bounds__(kBlock) k_sah_level(SahArgs A)
     const uint32_t nleft = (uint32_t)s_split[2], nright = cnt - nleft;
     // pass 3: stable partition into ids_out, chunk by chunk
     uint32_t lbase = 0, rbase = 0;
-    for (uint32_t base = lo; base < hi; base += kBlock) {
+    for (uint32_t base = lo; base < hi; base += BLOCK) {
         const uint32_t i = base + tid;
         const bool valid = i < hi;
         uint32_t p = 0;
@@ -564,7 +588,7 @@ __global__ void __launch_bounds__(kBlock) k_sah_level(SahArgs A)
         if (lane == 0) s_wave[wave] = (uint32_t)__popcll(m);
         __syncthreads();
         uint32_t woff = 0, total = 0;
-        for (uint32_t w = 0; w < kBlock / 64; ++w) {
+        for (uint32_t w = 0; w < BLOCK / 64; ++w) {
             woff += w < wave ? s_wave[w] : 0u;
             total += s_wave[w];
         }
@@ -586,7 +610,7 @@ __global__ void __launch_bounds__(kBlock) k_sah_level(SahArgs A)
             }
         }
         lbase += total;
-        rbase += min((uint32_t)kBlock, hi - base) - total;
+        rbase += min((uint32_t)BLOCK, hi - base) - total;
         __syncthreads();   // s_wave is rewritten by the next chunk
     }
     if (tid == 0) {
@@ -598,14 +622,15 @@ __global__ void __launch_bounds__(kBlock) k_sah_level(SahArgs A)
                 child = n - 1u + start;
             } else {
                 child = atomicAdd(A.node_alloc, 1u);
-                const uint32_t slot = atomicAdd(A.ntasks_out, 1u);
-                if (child >= n - 1u || slot >= A.task_cap || m == 0u) {
+                const bool big = m >= A.big_min;
+                const uint32_t slot = atomicAdd(big ? A.nbig_out : A.ntasks_out, 1u);
+                if (child >= n - 1u || slot >= (big ? A.big_cap : A.task_cap) || m == 0u) {
                     atomicAdd(A.err, 1u);
                     child = 0u;
                 } else {
                     A.rfirst[child] = start;
                     A.ncount[child] = m;
-                    A.tasks_out[slot] = make_uint2(start, child);
+                    (big ? A.big_out : A.tasks_out)[slot] = make_uint2(start, child);
                 }
             }
             if (side) A.right[node] = child;
@@ -614,14 +639,17 @@ __global__ void __launch_bounds__(kBlock) k_sah_level(SahArgs A)
     }
 }
 
-__global__ void k_sah_init(uint32_t n, uint2 *task0, uint32_t *ncount, uint32_t *rfirst, uint32_t *counters)
+// the root task, in the big list (big) or the small one
+__global__ void k_sah_init(uint32_t n, bool big, uint2 *small0, uint2 *big0, uint32_t *ncount, uint32_t *rfirst,
+                           uint32_t *counters)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        task0[0] = make_uint2(0u, 0u);
+        (big ? big0 : small0)[0] = make_uint2(0u, 0u);
         ncount[0] = n;
         rfirst[0] = 0u;
-        counters[6] = 0u;   // tasks of the next level
+        counters[6] = 0u;   // small tasks of the next level
         counters[7] = 1u;   // internal nodes allocated (the root)
+        counters[8] = 0u;   // big tasks of the next level
     }
 }
 
@@ -766,9 +794,10 @@ void DevBuf::release()
 
 static int build_binary_lbvh(hipStream_t s, uint32_t n, DevScratch &sc, const uint32_t *&vals_out);
 
-// Binary tree by binned SAH (k_sah_level), one launch and one read-back of
-// the next level's task count per level.  Order buffers: vals[0] / vals[1];
-// leaf order: keys[0]; tasks: flags / frontier (at most n / 2 per level).
+// Binary tree by binned SAH (k_sah_level), one or two launches (small and big
+// nodes) and one read-back of the next level's task counts per level.  Order
+// buffers: vals[0] / vals[1]; leaf order: keys[0]; small tasks: flags /
+// frontier (at most n / 2 per level); big tasks: the two halves of hist.
 static int build_binary_sah(hipStream_t s, uint32_t n, DevScratch &sc, const uint32_t *&vals_out)
 {
     uint32_t *counters = (uint32_t *)sc.counters.ptr;
@@ -776,21 +805,29 @@ static int build_binary_sah(hipStream_t s, uint32_t n, DevScratch &sc, const uin
     uint32_t *ids[2] = {(uint32_t *)sc.vals[0].ptr, (uint32_t *)sc.vals[1].ptr};
     uint32_t *vals = (uint32_t *)sc.keys[0].ptr;
     uint2 *tasks[2] = {(uint2 *)sc.flags.ptr, (uint2 *)sc.frontier.ptr};
+    const uint32_t big_cap = (uint32_t)std::min<uint64_t>(n / kSahBigMin + 1, sc.hist.bytes / 16);
+    uint2 *bigs[2] = {(uint2 *)sc.hist.ptr, (uint2 *)sc.hist.ptr + big_cap};
     uint32_t *ncount = (uint32_t *)sc.ncount.ptr, *rfirst = (uint32_t *)sc.rfirst.ptr;
     const uint32_t cap = n / 2;
+    if (big_cap < 1) {
+        set_error("SAH BVH build: no room for the task lists");
+        return SOLHIP_ERROR_DEVICE;
+    }
+    const bool root_big = n >= kSahBigMin;
     hipLaunchKernelGGL(k_iota, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, ids[0], n);
-    hipLaunchKernelGGL(k_sah_init, dim3(1), dim3(64), 0, s, n, tasks[0], ncount, rfirst, counters);
-    uint32_t ntasks = 1, level = 0;
+    hipLaunchKernelGGL(k_sah_init, dim3(1), dim3(64), 0, s, n, root_big, tasks[0], bigs[0], ncount, rfirst, counters);
+    uint32_t nsmall = root_big ? 0u : 1u, nbig = root_big ? 1u : 0u, level = 0;
     int cur = 0;
-    while (ntasks) {
+    while (nsmall || nbig) {
         SahArgs a;
         a.boxes = (const float4 *)sc.boxes.ptr;
         a.ids_in = ids[cur];
         a.ids_out = ids[cur ^ 1];
         a.vals = vals;
-        a.tasks_in = tasks[cur];
         a.tasks_out = tasks[cur ^ 1];
+        a.big_out = bigs[cur ^ 1];
         a.ntasks_out = counters + 6;
+        a.nbig_out = counters + 8;
         a.node_alloc = counters + 7;
         a.left = (uint32_t *)sc.left.ptr;
         a.right = (uint32_t *)sc.right.ptr;
@@ -799,22 +836,33 @@ static int build_binary_sah(hipStream_t s, uint32_t n, DevScratch &sc, const uin
         a.nbox = (float4 *)sc.nbox.ptr;
         a.n = n;
         a.task_cap = cap;
+        a.big_cap = big_cap;
+        a.big_min = kSahBigMin;
         a.level = level;
         a.err = counters + 3;
-        hipLaunchKernelGGL(k_sah_level, dim3(ntasks), dim3(kBlock), 0, s, a);
-        SOLHIP_HIP_CHECK(hipMemcpyAsync(pinned, counters, 32, hipMemcpyDeviceToHost, s));
+        if (nbig) {
+            a.tasks_in = bigs[cur];
+            hipLaunchKernelGGL(k_sah_level<1024>, dim3(nbig), dim3(1024), 0, s, a);
+        }
+        if (nsmall) {
+            a.tasks_in = tasks[cur];
+            hipLaunchKernelGGL(k_sah_level<kBlock>, dim3(nsmall), dim3(kBlock), 0, s, a);
+        }
+        SOLHIP_HIP_CHECK(hipMemcpyAsync(pinned, counters, 36, hipMemcpyDeviceToHost, s));
         SOLHIP_HIP_CHECK(hipStreamSynchronize(s));
         if (pinned[3]) {
             set_error("SAH BVH build range check failed at level " + std::to_string(level));
             return SOLHIP_ERROR_DEVICE;
         }
-        ntasks = pinned[6];
+        nsmall = pinned[6];
+        nbig = pinned[8];
         ++level;
-        if (ntasks > cap || level > kSahMedianLevel + 40) {
+        if (nsmall > cap || nbig > big_cap || level > kSahMedianLevel + 40) {
             set_error("SAH BVH build did not converge");
             return SOLHIP_ERROR_DEVICE;
         }
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, counters + 6, 1u, 0u);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, counters + 8, 1u, 0u);
         cur ^= 1;
     }
     if (pinned[7] != n - 1) {

@@ -137,48 +137,55 @@ struct SahSplit {
 };
 
 // Bins: count(a, b), lo(a, b, k), hi(a, b, k) (empty bins: +inf / -inf).
+// Best split along axis a; updates (best_cost, best) only on a strictly lower
+// cost, so running the axes in order x, y, z (or each alone and keeping the
+// first strict minimum in that order) keeps the first split found.
+template <class Bins>
+SOLHIP_HD void sah_axis(const Bins &B, int a, float &best_cost, SahSplit &best)
+{
+    float ra[kSahBins];
+    uint32_t rc[kSahBins];
+    float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+    uint32_t c = 0;
+    for (int b = kSahBins - 1; b > 0; --b) {
+        c += B.count(a, b);
+        for (int k = 0; k < 3; ++k) {
+            l[k] = fminf(l[k], B.lo(a, b, k));
+            h[k] = fmaxf(h[k], B.hi(a, b, k));
+        }
+        ra[b] = c ? half_area3(l, h) : 0.0f;
+        rc[b] = c;
+    }
+    for (int k = 0; k < 3; ++k) {
+        l[k] = INFINITY;
+        h[k] = -INFINITY;
+    }
+    c = 0;
+    for (int b = 0; b < kSahBins - 1; ++b) {
+        c += B.count(a, b);
+        for (int k = 0; k < 3; ++k) {
+            l[k] = fminf(l[k], B.lo(a, b, k));
+            h[k] = fmaxf(h[k], B.hi(a, b, k));
+        }
+        if (c == 0 || rc[b + 1] == 0) continue;
+        const float cost = half_area3(l, h) * (float)c + ra[b + 1] * (float)rc[b + 1];
+        if (cost < best_cost) {
+            best_cost = cost;
+            best.axis = a;
+            best.bin = b;
+            best.nleft = c;
+        }
+    }
+}
+
 template <class Bins>
 SOLHIP_HD SahSplit sah_choose(const Bins &B, const float scale[3], uint32_t count, uint32_t level)
 {
     SahSplit best{-1, 0, count / 2};
     if (level >= kSahMedianLevel) return best;
     float best_cost = INFINITY;
-    for (int a = 0; a < 3; ++a) {
-        if (!(scale[a] > 0.0f)) continue;
-        float ra[kSahBins];
-        uint32_t rc[kSahBins];
-        float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
-        uint32_t c = 0;
-        for (int b = kSahBins - 1; b > 0; --b) {
-            c += B.count(a, b);
-            for (int k = 0; k < 3; ++k) {
-                l[k] = fminf(l[k], B.lo(a, b, k));
-                h[k] = fmaxf(h[k], B.hi(a, b, k));
-            }
-            ra[b] = c ? half_area3(l, h) : 0.0f;
-            rc[b] = c;
-        }
-        for (int k = 0; k < 3; ++k) {
-            l[k] = INFINITY;
-            h[k] = -INFINITY;
-        }
-        c = 0;
-        for (int b = 0; b < kSahBins - 1; ++b) {
-            c += B.count(a, b);
-            for (int k = 0; k < 3; ++k) {
-                l[k] = fminf(l[k], B.lo(a, b, k));
-                h[k] = fmaxf(h[k], B.hi(a, b, k));
-            }
-            if (c == 0 || rc[b + 1] == 0) continue;
-            const float cost = half_area3(l, h) * (float)c + ra[b + 1] * (float)rc[b + 1];
-            if (cost < best_cost) {
-                best_cost = cost;
-                best.axis = a;
-                best.bin = b;
-                best.nleft = c;
-            }
-        }
-    }
+    for (int a = 0; a < 3; ++a)
+        if (scale[a] > 0.0f) sah_axis(B, a, best_cost, best);
     return best;
 }
 
