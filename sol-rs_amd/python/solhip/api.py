@@ -331,7 +331,9 @@ class Context:
 class GpuScene:
     """Uploads a SceneData (sol-rs_amd/python/solhip/scenes.py) and builds its
     BLASes and TLAS: what SceneDescription::from_scene does for the Vulkan
-    path (src/ray/mod.rs:59-156)."""
+    path (src/ray/mod.rs:59-156).  build_flags: the BLAS build flags
+    (BUILD_PREFER_FAST_TRACE: binned SAH, as BLAS::new asks of Vulkan; 0:
+    Morton order, the fast build for geometry rebuilt every frame)."""
 
     def __init__(self, ctx, sd, index_size=4, build_flags=0):
         self.ctx = ctx
