@@ -253,7 +253,19 @@ __global__ void __launch_bounds__(kSrvBlock) SOLHIP_SRV_OCCUPANCY k_srv_shadow(S
     srv_trace<true>(A, A.sq, A.counts[2], A.heads + kSrvParts * kHeadStride, SOLHIP_KERNEL_SRV_SHADOW);
 }
 
-__global__ void __launch_bounds__(kSrvShadeBlock) k_srv_shade(SrvArgs A, uint32_t cur)
+// waves per SIMD the shade kernel is compiled for (0: the compiler's choice,
+// 126 VGPRs and 4 waves; 5 fits in 96 VGPRs, but the 40-request load is
+// level with it: p50 1352-1354 vs 1356 ms)
+#ifndef SOLHIP_SRV_SHADE_WAVES
+#define SOLHIP_SRV_SHADE_WAVES 0
+#endif
+#if SOLHIP_SRV_SHADE_WAVES
+#define SOLHIP_SRV_SHADE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SOLHIP_SRV_SHADE_WAVES)))
+#else
+#define SOLHIP_SRV_SHADE_OCCUPANCY
+#endif
+
+__global__ void __launch_bounds__(kSrvShadeBlock) SOLHIP_SRV_SHADE_OCCUPANCY k_srv_shade(SrvArgs A, uint32_t cur)
 {
     __shared__ uint32_t lds[kSrvShadeWaves + 1];
     const uint32_t cnt = A.counts[cur];

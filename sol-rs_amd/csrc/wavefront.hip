@@ -297,8 +297,20 @@ __global__ void SOLHIP_WF_OCCUPANCY k_pt_shadow(WfArgs A, uint32_t b)
 // 4223 Mrays/s); without them 2-wave blocks: larger blocks wait longer at the
 // append barriers on scattered scenes (10k instances: 1259 with 1 wave, 1277
 // with 2, 1225 with 8).
+// waves per SIMD the shade kernel is compiled for (0: the compiler's choice,
+// 115 VGPRs and 4 waves; 5 fits in 90 VGPRs without spills: Cornell box
+// 4176 -> 4287 Mrays/s, 10k instances 1252 -> 1258; 6 spills and is slower)
+#ifndef SOLHIP_WF_SHADE_WAVES
+#define SOLHIP_WF_SHADE_WAVES 5
+#endif
+#if SOLHIP_WF_SHADE_WAVES
+#define SOLHIP_WF_SHADE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SOLHIP_WF_SHADE_WAVES)))
+#else
+#define SOLHIP_WF_SHADE_OCCUPANCY
+#endif
+
 template <uint32_t WAVES>
-__global__ void __launch_bounds__(64 * WAVES) k_pt_shade(WfArgs A, uint32_t b)
+__global__ void __launch_bounds__(64 * WAVES) SOLHIP_WF_SHADE_OCCUPANCY k_pt_shade(WfArgs A, uint32_t b)
 {
     __shared__ uint32_t lds[WAVES + 1];
     const uint32_t cnt = A.counts[b & 1];
