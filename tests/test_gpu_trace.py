@@ -145,13 +145,17 @@ def test_heightfield_large_sampled(ctx, oracle, build):
 
 
 @BUILDS
-@pytest.mark.parametrize("name", ["soup", "instances", "heightfield128"])
+@pytest.mark.parametrize("name", ["soup", "instances", "heightfield128", "heightfield256"])
 def test_gpu_tree_equals_host_replay(ctx, oracle, name, build):
     """the GPU build makes the tree the host replay of the same steps makes
     (tests/cpu_harness): same hits and the same node, triangle and instance
-    fetch counts over a ray set"""
+    fetch counts over a ray set (heightfield256: 131,072 triangles, so the
+    SAH build's top nodes run on its 1024-thread workgroups)"""
     from cpu_harness import harness
-    sd = scenes.heightfield(128) if name == "heightfield128" else SMALL_SCENES[name]()
+    if name.startswith("heightfield"):
+        sd = scenes.heightfield(int(name[len("heightfield"):]))
+    else:
+        sd = SMALL_SCENES[name]()
     rays = np.concatenate([raygen.camera_rays(sd, 64, 36, oracle), raygen.random_rays(3000, (0, 0, 0), 3.0, seed=4)])
     gs = solhip.GpuScene(ctx, sd, build_flags=build)
     ctx.stats(reset=True)
