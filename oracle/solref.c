@@ -5,7 +5,11 @@
  * the shipped HIP library never links or loads it.
  *
  * What pins each part of this restatement (checked in tests/):
- *   sr_tea, sr_next_rand ....... golden streams written by an independent
+ *   sr_tea ..................... TEA (Wheeler and Needham 1994) with 16 cycles
+ *                                and the shader's key: a cipher pinned by the
+ *                                published 32-cycle zero-key vector
+ *                                (41ea3a0a 94baa940) gives the known answers.
+ *   sr_next_rand ............... golden streams written by an independent
  *                                numpy-uint32 restatement
  *                                (tests/golden/make_golden.py) and the PCG
  *                                RXS-M-XS 32/32 output function (O'Neill 2014);
@@ -20,9 +24,26 @@
  *   sr_intersect_triangle ...... exact hits on axis-aligned quads, edge
  *                                watertightness, t = distance on known cases.
  *   sr_trace(use_bvh=1) ........ equality with brute force (use_bvh=0).
- *   renderers .................. closed-form scenes (empty scene -> AO 1 /
- *                                black, infinite plane AO expectation,
- *                                furnace-style constant sky) and the GPU.
+ *   sr_render_ao ............... an open plane gives exactly 1; a floor
+ *                                under a parallel ceiling gives occlusion
+ *                                1 - (h/R)^2.
+ *   sr_render_ao_glsl .......... an open plane gives exactly 1 (the chain
+ *                                adds from its second hit on); between two
+ *                                parallel planes the image mean is
+ *                                1 - E[K]/4 with P(K >= j) = p^j, p the
+ *                                share of blue-noise draws whose cosine
+ *                                direction reaches the other plane.
+ *   sr_render_pathtrace_glsl ... sky gradient at zenith / nadir; emitters
+ *                                seen directly; a roughness-0 metal plane
+ *                                gives base * sky(reflect(d, n)) per pixel;
+ *                                a roughness-0 black dielectric reflects
+ *                                with probability F = 0.04.
+ *   sr_render_pathtrace ........ bit-identical to sr_render_pathtrace_glsl
+ *                                without extensions; NEE / roulette /
+ *                                per-sample streams keep the expectation.
+ *   sr_render_debug ............ barycentrics of the ray/plane point solved
+ *                                in float64; misses return the camera ray;
+ *                                nothing beyond the range 1000 is seen.
  */
 #include "solref.h"
 

@@ -25,11 +25,34 @@ def test_tea_and_pcg_match_golden_vectors(oracle):
         assert out.view(np.uint32).tolist() == fl.view(np.uint32).tolist()
 
 
-def test_tea_known_answer(oracle):
-    # TEA-16 of (0, 0): fixed by the algorithm alone
-    from golden.make_golden import tea
-    assert oracle.tea(0, 0) == int(tea(np.array([0], np.uint32), np.array([0], np.uint32))[0])
-    assert oracle.tea(0, 0) != oracle.tea(1, 0) != oracle.tea(0, 1)
+def _tea_cipher(v0, v1, key, cycles):
+    """the Tiny Encryption Algorithm (Wheeler and Needham 1994) on Python
+    integers modulo 2^32, with its key and cycle count as parameters"""
+    M = 0xFFFFFFFF
+    s = 0
+    for _ in range(cycles):
+        s = (s + 0x9E3779B9) & M
+        v0 = (v0 + ((((v1 << 4) & M) + key[0]) & M ^ (v1 + s) & M ^ ((v1 >> 5) + key[1]) & M)) & M
+        v1 = (v1 + ((((v0 << 4) & M) + key[2]) & M ^ (v0 + s) & M ^ ((v0 >> 5) + key[3]) & M)) & M
+    return v0, v1
+
+
+def test_tea_cipher_matches_the_published_test_vector():
+    # the standard 32-cycle TEA with an all-zero key encrypts the zero block
+    # to 41ea3a0a 94baa940: this pins the parametrised cipher above
+    assert _tea_cipher(0, 0, (0, 0, 0, 0), 32) == (0x41EA3A0A, 0x94BAA940)
+
+
+def test_tea_is_the_16_cycle_cipher_with_the_shader_key(oracle):
+    """sampling.glsl:18-32 is TEA with 16 cycles and the key (0xa341316c,
+    0xc8013ea4, 0xad90777d, 0x7e95761e), returning the first word: known
+    answers from the cipher pinned by the published vector"""
+    key = (0xA341316C, 0xC8013EA4, 0xAD90777D, 0x7E95761E)
+    rng = np.random.default_rng(11)
+    cases = [(0, 0), (1, 0), (0, 1), (0xFFFFFFFF, 0xFFFFFFFF)] + [tuple(int(x) for x in rng.integers(0, 2 ** 32, 2))
+                                                                   for _ in range(64)]
+    for a, b in cases:
+        assert oracle.tea(a, b) == _tea_cipher(a, b, key, 16)[0]
 
 
 def test_next_rand_is_word_times_2_pow_minus_32(oracle):
@@ -411,3 +434,179 @@ def test_depth_zero_sees_only_emitters_and_sky(oracle):
     assert np.array_equal(a, b)
     vals = np.unique(a[..., 0])
     assert set(np.round(vals * 4).astype(int)) <= {0, 15, 30, 45, 60}   # k of 4 samples saw the light
+
+
+# ---------------------------------------------------------------------------
+# the GLSL programs' own chains: ao.* (assets/glsl/ao.rgen:37-83,
+# ao.rchit:45-88), pathtrace.rchit's lobe choice (pathtrace.rchit:91-112) and
+# debug.* (debug.rgen:18-37, debug.rchit:9-13, debug.rmiss:6-9)
+# ---------------------------------------------------------------------------
+SKY_TOP = np.array([0.3, 0.4, 0.5])
+SKY_BOTTOM = np.array([0.58, 0.45, 0.25])
+
+
+def _sky(dirs):
+    """pathtrace.rmiss:9-17 in float64 for unit directions (N, 3), without the
+    sun (the tests aim away from it and assert so)"""
+    dirs = np.asarray(dirs, np.float64)
+    sun = np.array([0.0, 1.0, -0.25]) / np.linalg.norm([0.0, 1.0, -0.25])
+    assert (dirs @ sun < 0.98).all()
+    x = np.clip((0.5 * (dirs[:, 1] + 1.0) - 0.35) / 0.3, 0.0, 1.0)
+    t = x * x * (3.0 - 2.0 * x)
+    return SKY_BOTTOM * (1.0 - t[:, None]) + SKY_TOP * t[:, None]
+
+
+def _camera_dirs(u, W, H, jx, jy):
+    """camera rays of pathtrace.rgen:52-58 in float64 from the uniforms, for
+    sub-pixel offsets (jx, jy): (H, W, 3) unit directions and the origin"""
+    vi = np.asarray(u["view_inverse"][0], np.float64).reshape(4, 4).T
+    pi = np.asarray(u["projection_inverse"][0], np.float64).reshape(4, 4).T
+    xs, ys = np.meshgrid(np.arange(W) + jx, np.arange(H) + jy)
+    ndc = np.stack([xs / W * 2.0 - 1.0, ys / H * 2.0 - 1.0, np.ones_like(xs), np.ones_like(xs)], -1)
+    tgt = ndc @ pi.T
+    t3 = tgt[..., :3] / np.linalg.norm(tgt[..., :3], axis=-1, keepdims=True)
+    d = t3 @ vi[:3, :3].T
+    return vi[:3, 3], d
+
+
+def _blue_permutations(seed=5):
+    """255 x 1 RGBA8 texture whose every channel is a permutation of
+    1..255: a blue-noise draw at a uniform texel is uniform over k/255,
+    k = 1..255, and never 0 (a cosine direction is then never tangent)"""
+    rng = np.random.default_rng(seed)
+    tex = np.zeros((1, 255, 4), np.uint8)
+    for c in range(4):
+        tex[0, :, c] = rng.permutation(255) + 1
+    return tex
+
+
+def test_ao_glsl_chain_on_an_open_plane_is_one(oracle):
+    """ao.rchit adds 1 only from the second hit of a chain on (depth > 0,
+    ao.rchit:84-86): over an open plane every chain is one hit and then an
+    escape, so the image is exactly 1"""
+    b = scenes.soup_blas(_plane(40.0))
+    sd = scenes.assemble("plane", [b], [0], [np.eye(4)], [0], [scenes.material((1, 1, 1))],
+                         scenes.Camera((0.0, 2.0, 2.0), (0.0, 0.0, 0.0)), 2.0)
+    W = H = 24
+    acc = np.zeros((H, W, 4), np.float32)
+    oracle.Scene(sd).render_ao_glsl(sd.uniforms(W, H, 0), 0, acc, _blue_permutations(), W, H)
+    assert (acc[..., :3] == 1.0).all()
+
+
+def test_ao_glsl_chain_between_parallel_planes_matches_closed_form(oracle):
+    """Floor at y = 0 and ceiling at y = h = 5, the camera between them
+    looking down.  A chain segment leaves one plane with cos(theta) =
+    sqrt(xi), xi = k/255 a uniform blue-noise draw (ao.rchit:45-51, 82;
+    sampling.glsl:93-96), and reaches the other plane within the range 10
+    (ao.rchit:83) iff cos(theta) >= h/10 = 0.5, i.e. k >= 64: p = 192/255.
+    The chain stops after 5 hits (ao.rgen:71, max_samples = 4), so the hits
+    after the first number K in 0..4 with P(K >= j) = p^j, and
+    E[colour] = 1 - E[K]/4 (ao.rgen:74-76)."""
+    h = 5.0
+    floor = scenes.soup_blas(_plane(200.0, 0.0))
+    ceil = scenes.soup_blas(_plane(200.0, h))
+    sd = scenes.assemble("slab", [floor, ceil], [0, 1], [np.eye(4)] * 2, [0, 0], [scenes.material((1, 1, 1))],
+                         scenes.Camera((0.0, 2.5, 0.0), (0.0, 0.0, 0.001), up=(0.0, 0.0, -1.0), vfov_deg=30.0),
+                         2.0)
+    W = H = 64
+    acc = np.zeros((H, W, 4), np.float32)
+    oracle.Scene(sd).render_ao_glsl(sd.uniforms(W, H, 0), 0, acc, _blue_permutations(), W, H)
+    k = np.arange(1, 256)
+    assert not np.any(np.abs(np.sqrt(k / 255.0) - 0.5) < 1e-4)   # no draw sits on the threshold
+    p = np.count_nonzero(np.sqrt(k / 255.0) >= 0.5) / 255.0
+    pk = np.array([p ** j * (1 - p) for j in range(4)] + [p ** 4])   # P(K = 0..4)
+    ek = (np.arange(5) * pk).sum()
+    vk = (np.arange(5) ** 2 * pk).sum() - ek ** 2
+    col = acc[..., 0].astype(np.float64)
+    # a pixel's colour is 1 - (sum of its 4 samples' K) / 16
+    assert np.allclose(np.round((1.0 - col) * 16.0), (1.0 - col) * 16.0, atol=1e-5)
+    se = math.sqrt(vk / (W * H * 4)) / 4.0
+    assert abs(col.mean() - (1.0 - ek / 4.0)) < 5.0 * se, (col.mean(), 1.0 - ek / 4.0, se)
+
+
+def _mirror_plane_scene(metallic, roughness, base, eye, target, vfov):
+    b = scenes.soup_blas(_plane(500.0))
+    return scenes.assemble("mirror", [b], [0], [np.eye(4)], [0],
+                           [scenes.material(base, metallic=metallic, roughness=roughness)],
+                           scenes.Camera(eye, target, up=(0.0, -1.0, 0.0), vfov_deg=vfov), 2.0)
+
+
+def test_pathtrace_glsl_metal_mirror_reflects_the_sky(oracle):
+    """metallic 1, roughness 0 (pathtrace.rchit:97-100): every path takes the
+    metal lobe, GGX with alpha = 0 returns reflect(d, n) exactly
+    (sampling.glsl:88-91), and the reflected ray escapes to the sky, so a
+    pixel is base * vertex colour * sky(reflect(d, n)).  The camera looks
+    almost horizontally over the plane, where the sky gradient is steep: each
+    pixel lies between the closed form at its footprint's upper and lower
+    edges."""
+    base = (0.9, 0.6, 0.3)
+    sd = _mirror_plane_scene(1.0, 0.0, base, (0.0, 1.0, 0.0), (0.0, 0.8, 1.0), 12.0)
+    W = H = 48
+    u = sd.uniforms(W, H, 0)
+    acc = np.zeros((H, W, 4), np.float32)
+    oracle.Scene(sd).render_pathtrace_glsl(u, oracle.PTParams(1, 8, 32), 0, acc, None, W, H)
+    _, d0 = _camera_dirs(u, W, H, 0.0, 0.0)
+    _, d1 = _camera_dirs(u, W, H, 1.0, 1.0)
+    assert (d0[..., 1] < 0).all() and (d1[..., 1] < 0).all()   # every ray hits the plane
+    refl = lambda d: d * np.array([1.0, -1.0, 1.0])
+    s0 = _sky(refl(d0).reshape(-1, 3)).reshape(H, W, 3) * base
+    s1 = _sky(refl(d1).reshape(-1, 3)).reshape(H, W, 3) * base
+    lo, hi = np.minimum(s0, s1), np.maximum(s0, s1)
+    img = acc[..., :3].astype(np.float64)
+    assert ((img >= lo - 2e-6) & (img <= hi + 2e-6)).all()
+    assert (hi - lo).max() < 0.02 and img[..., 0].std() > 0.01   # tight brackets, steep gradient
+
+
+def test_pathtrace_glsl_dielectric_coat_reflects_with_fresnel_004(oracle):
+    """metallic 0, roughness 0, black base (pathtrace.rchit:101-111): the
+    microfacet normal is the surface normal, so the coat reflects with
+    probability F(n, n, 1/1.5) = 0.04 (a mirror ray to the sky, value 1 *
+    sky) and otherwise takes the diffuse lobe with value base = 0.  The
+    camera looks steeply down, so every reflection sees the top sky colour:
+    a pixel is k/8 of it (k of 8 samples reflected), and the mean of k/8 is
+    0.04 within 5 standard errors."""
+    sd = _mirror_plane_scene(0.0, 0.0, (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.2, 0.6), 20.0)
+    W = H = 64
+    u = sd.uniforms(W, H, 0)
+    _, d0 = _camera_dirs(u, W, H, 0.5, 0.5)
+    assert (-d0[..., 1] > 0.35).all()   # reflections above the gradient band: sky = top colour
+    acc = np.zeros((H, W, 4), np.float32)
+    oracle.Scene(sd).render_pathtrace_glsl(u, oracle.PTParams(1, 8, 32), 0, acc, None, W, H)
+    frac = acc[..., :3].astype(np.float64) / SKY_TOP
+    k8 = frac * 8.0
+    assert np.allclose(k8, np.round(k8), atol=1e-4)
+    assert np.allclose(frac[..., 0], frac[..., 1], atol=1e-5) and np.allclose(frac[..., 0], frac[..., 2], atol=1e-5)
+    n = W * H * 8
+    se = math.sqrt(0.04 * 0.96 / n)
+    assert abs(frac[..., 0].mean() - 0.04) < 5.0 * se, (frac[..., 0].mean(), se)
+
+
+def test_debug_program_barycentrics_and_misses(oracle):
+    """debug.rgen:18-37: one ray through each pixel centre, range
+    [0.001, 1000]; a hit stores (1-u-v, u, v) (debug.rchit:11-12), a miss
+    keeps the ray direction (debug.rmiss leaves the payload).  Closed form:
+    the barycentrics of the ray/plane intersection solved in float64 from
+    the triangle's vertices; misses equal the float64 camera ray; a copy of
+    the triangle beyond the range is never seen."""
+    v = np.array([(-1.2, -0.9, 0.0), (1.4, -1.0, 0.0), (-0.3, 1.3, 0.0)], np.float64)
+    far = v + np.array([0.0, 0.0, -2000.0])
+    near = scenes.soup_blas(v.astype(np.float32))
+    beyond = scenes.soup_blas(far.astype(np.float32) * np.array([600.0, 600.0, 1.0], np.float32))
+    sd = scenes.assemble("debug", [near, beyond], [0, 1], [np.eye(4)] * 2, [0, 0], [scenes.material((1, 1, 1))],
+                         scenes.Camera((0.1, 0.2, 3.0), (0.0, 0.0, 0.0), vfov_deg=60.0), 2.0)
+    W, H = 40, 30
+    u = sd.uniforms(W, H, 0)
+    out = oracle.Scene(sd).render_debug(u, W, H)
+    o, d = _camera_dirs(u, W, H, 0.5, 0.5)
+    t = -o[2] / d[..., 2]
+    P = o + t[..., None] * d
+    A = np.array([[v[0, 0], v[1, 0], v[2, 0]], [v[0, 1], v[1, 1], v[2, 1]], [1.0, 1.0, 1.0]])
+    rhs = np.stack([P[..., 0], P[..., 1], np.ones_like(t)], -1)
+    bary = np.linalg.solve(A, rhs.reshape(-1, 3).T).T.reshape(H, W, 3)
+    inside = (bary > 1e-4).all(-1)
+    outside = (bary < -1e-4).any(-1)
+    assert inside.sum() > 200 and outside.sum() > 200
+    assert np.allclose(out[inside][:, :3], bary[inside], atol=2e-5)
+    assert np.allclose(out[inside][:, :3].sum(-1), 1.0, atol=1e-6)
+    assert np.allclose(out[outside][:, :3], d[outside], atol=1e-6)
+    assert (out[..., 3] == 0.0).all()
