@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define SOLHIP_ABI_VERSION 1u
+#define SOLHIP_ABI_VERSION 2u
 
 typedef uint64_t solhip_ctx;
 typedef uint64_t solhip_stream;
@@ -52,7 +52,10 @@ enum {
     SOLHIP_ERROR_NO_DEVICE = -4,
     SOLHIP_ERROR_DEVICE = -5,        /* HIP runtime / kernel failure */
     SOLHIP_ERROR_UNSUPPORTED = -6,
-    SOLHIP_ERROR_OVERFLOW = -7       /* a device-side capacity was exceeded */
+    SOLHIP_ERROR_OVERFLOW = -7,      /* a device-side capacity was exceeded */
+    SOLHIP_ERROR_STALE = -8          /* a TLAS (or a server scene) refers to a BLAS / TLAS
+                                        that was rebuilt or destroyed after it was built:
+                                        rebuild it (the rule Vulkan has for its TLAS) */
 };
 
 /* ---- records with the reference's byte layouts ------------------------- */
@@ -97,29 +100,36 @@ typedef struct {
 } solhip_scene_uniforms;
 
 /* BLAS geometry: replaces vk::AccelerationStructureGeometryTrianglesDataKHR +
- * BuildRangeInfo as BLAS::new fills them (src/ray/acceleration.rs:148-208):
- * positions are R32G32B32_SFLOAT at byte offset 0 of each vertex, vertex i of
- * a triangle is at vertex_addr + (first_vertex + index) * vertex_stride, and
- * indices are UINT32 at index_addr + index_byte_offset. */
+ * BuildRangeInfo as BLAS::new fills them (src/ray/acceleration.rs:148-208).
+ * Positions are R32G32B32_SFLOAT at byte offset 0 of each vertex.
+ *   indexed (index_addr != 0): primitive_count = index_count / 3; the indices
+ *     are UINT32 at index_addr + primitive_offset, and vertex k of a triangle
+ *     is at vertex_addr + (first_vertex + index_k) * vertex_stride; every
+ *     index must be < vertex_count (BLAS::new's max_vertex + 1);
+ *   non-indexed (index_addr == 0, index_count == 0): primitive_count =
+ *     vertex_count / 3 (acceleration.rs:185-188) and vertex k of triangle i is
+ *     at vertex_addr + primitive_offset + (first_vertex + 3 i + k) * vertex_stride,
+ *     VkAccelerationStructureBuildRangeInfoKHR's rule for non-indexed data. */
 typedef struct {
     uint64_t vertex_addr;        /* device address (solhip_buffer_address) */
-    uint64_t vertex_stride;      /* 64 for ModelVertex */
+    uint64_t vertex_stride;      /* 64 for ModelVertex; >= 12, a multiple of 4 */
     uint32_t first_vertex;
     uint32_t vertex_count;
-    uint64_t index_addr;         /* device address of a u32 index buffer */
-    uint64_t index_byte_offset;
-    uint32_t index_count;        /* primitive_count = index_count / 3 */
+    uint64_t index_addr;         /* device address of a u32 index buffer, or 0 */
+    uint64_t primitive_offset;   /* BuildRangeInfo::primitive_offset, bytes */
+    uint32_t index_count;        /* indexed: a multiple of 3; non-indexed: 0 */
     uint32_t opaque;             /* GeometryFlagsKHR::OPAQUE; must be 1 */
 } solhip_geometry;
 
-/* ray / hit records of solhip_trace_rays (32 B / 20 B).  Closest hit is the
- * lexicographic minimum of (t, instance index, primitive index) over all
- * triangles with t in [tmin, tmax]; on a miss t = -1, inst = prim =
- * 0xFFFFFFFF.  inst is the TLAS instance index (gl_InstanceID), prim the
- * triangle index in its BLAS (gl_PrimitiveID), (u, v) the barycentrics of
- * vertices 1 and 2 (hitAttributeEXT.xy). */
+/* ray / hit records of solhip_trace_rays (32 B / 24 B).  Closest hit is the
+ * lexicographic minimum of (t, instance index, geometry index, primitive
+ * index) over all triangles with t in [tmin, tmax]; on a miss t = -1 and
+ * inst = prim = geom = 0xFFFFFFFF.  inst is the TLAS instance index
+ * (gl_InstanceID), geom the geometry of the BLAS (gl_GeometryIndexEXT), prim
+ * the triangle index within that geometry (gl_PrimitiveID), (u, v) the
+ * barycentrics of vertices 1 and 2 (hitAttributeEXT.xy). */
 typedef struct { float ox, oy, oz, tmin, dx, dy, dz, tmax; } solhip_ray;
-typedef struct { float t, u, v; uint32_t inst, prim; } solhip_hit;
+typedef struct { float t, u, v; uint32_t inst, prim, geom; } solhip_hit;
 #define SOLHIP_MISS 0xFFFFFFFFu
 
 /* trace flags */
@@ -232,9 +242,18 @@ int32_t solhip_buffer_fill(solhip_ctx ctx, solhip_stream s, solhip_buffer b, uin
 int32_t solhip_buffer_address(solhip_ctx ctx, solhip_buffer b, uint64_t *addr);
 
 /* ---- acceleration structures ------------------------------------------- */
-/* BLAS::new (src/ray/acceleration.rs:136-239): GPU BVH build, enqueued on s.
- * Exactly one geometry per BLAS (SceneDescription::from_meshes builds one
- * BLAS per primitive, src/ray/mod.rs:125-132).  flags: 0 builds over Morton
+/* BLAS::new (src/ray/acceleration.rs:136-239): GPU BVH build over the n
+ * geometries g[0..n) (BLAS::new's Vec<GeometryInstance>), enqueued on s.
+ * Their triangles are numbered geometry after geometry; the hit shaders of
+ * the programs index an instance's index table with that BLAS-wide number,
+ * which is gl_PrimitiveID for the one-geometry BLAS that
+ * SceneDescription::from_meshes builds (src/ray/mod.rs:125-132).
+ * Invalid geometry (null addresses with triangles, stride < 12 or not a
+ * multiple of 4, an index_count that is not a multiple of 3, an index
+ * >= vertex_count) fails with SOLHIP_ERROR_INVALID_ARGUMENT.  A rebuild or
+ * destroy of a BLAS makes every TLAS built over it stale: traces through
+ * such a TLAS fail with SOLHIP_ERROR_STALE until it is rebuilt, as Vulkan
+ * requires a TLAS build after its BLASes change.  flags: 0 builds over Morton
  * order (fast build, for geometry rebuilt every frame);
  * SOLHIP_BUILD_PREFER_FAST_TRACE builds top down by binned SAH (slower build,
  * fewer node tests per ray), the flag BLAS::new passes to Vulkan
@@ -243,7 +262,8 @@ int32_t solhip_buffer_address(solhip_ctx ctx, solhip_buffer b, uint64_t *addr);
 #define SOLHIP_BUILD_PREFER_FAST_TRACE 0x1u
 int32_t solhip_blas_build(solhip_ctx ctx, solhip_stream s, const solhip_geometry *g, uint32_t n,
                           uint32_t flags, solhip_blas *out);
-/* rebuild in place from the same (or updated) vertex data */
+/* rebuild in place from the same (or updated) vertex data; n may differ
+ * from the first build */
 int32_t solhip_blas_rebuild(solhip_ctx ctx, solhip_stream s, solhip_blas b, const solhip_geometry *g, uint32_t n);
 int32_t solhip_blas_destroy(solhip_ctx ctx, solhip_blas b);
 /* TLAS::new / TLAS::regenerate (src/ray/acceleration.rs:344-467) */
@@ -362,7 +382,11 @@ typedef struct {
  * running jobs); max_jobs <= 256 running at once */
 int32_t solhip_server_create(solhip_ctx ctx, uint64_t hbm_budget, uint32_t max_jobs, solhip_server *out);
 int32_t solhip_server_destroy(solhip_ctx ctx, solhip_server srv);
-/* registers a resident scene: TLAS + hit-shader tables (+ its emitter table) */
+/* registers a resident scene: TLAS + hit-shader tables (+ its emitter table).
+ * Rebuilding or destroying the TLAS (or one of its BLASes) afterwards makes
+ * the scene stale: solhip_server_submit and solhip_server_run then fail with
+ * SOLHIP_ERROR_STALE while it has jobs, and the scene must be registered
+ * again under a new id. */
 int32_t solhip_server_scene(solhip_ctx ctx, solhip_server srv, solhip_tlas t, const solhip_scene_tables *tables,
                             uint32_t *scene_id);
 int32_t solhip_server_submit(solhip_ctx ctx, solhip_server srv, const solhip_job_desc *job, uint64_t *job_id);

@@ -356,10 +356,12 @@ typedef struct { float lo[3], hi[3]; uint32_t first, count; uint32_t left, right
 
 typedef struct {
     uint32_t ntri;
-    float *tri;          /* 9 floats per triangle, original primitive order */
+    float *tri;          /* 9 floats per triangle, BLAS-wide primitive order */
     bnode *nodes;
     uint32_t nnodes;
     uint32_t *order;     /* leaf ranges index into this permutation */
+    uint32_t ngeo;
+    uint32_t *geo_begin; /* first BLAS-wide triangle of each geometry */
 } blas_t;
 
 struct sr_scene {
@@ -479,23 +481,50 @@ static void build_bvh(const float (*lo)[3], const float (*hi)[3], uint32_t n, ui
     *nodes_out = b.nodes; *nnodes_out = b.nnodes; *order_out = order;
 }
 
+static uint32_t geo_tris(const sr_geometry *g)
+{
+    return g->indices ? g->index_count / 3 : g->vertex_count / 3;
+}
+
 sr_scene *sr_scene_create(const sr_geometry *g, uint32_t nblas,
                           const sr_instance_desc *inst, uint32_t ninst)
+{
+    uint32_t *ones = (uint32_t *)malloc(sizeof(uint32_t) * (nblas ? nblas : 1));
+    for (uint32_t b = 0; b < nblas; ++b) ones[b] = 1;
+    sr_scene *s = sr_scene_create_multi(g, ones, nblas, inst, ninst);
+    free(ones);
+    return s;
+}
+
+sr_scene *sr_scene_create_multi(const sr_geometry *geos, const uint32_t *geo_count, uint32_t nblas,
+                                const sr_instance_desc *inst, uint32_t ninst)
 {
     sr_scene *s = (sr_scene *)calloc(1, sizeof(sr_scene));
     s->nblas = nblas; s->ninst = ninst;
     s->blas = (blas_t *)calloc(nblas ? nblas : 1, sizeof(blas_t));
+    const sr_geometry *g = geos;
     for (uint32_t b = 0; b < nblas; ++b) {
         blas_t *bl = &s->blas[b];
-        uint32_t nt = g[b].index_count / 3;
+        const uint32_t ng = geo_count[b];
+        uint32_t nt = 0;
+        bl->ngeo = ng;
+        bl->geo_begin = (uint32_t *)malloc(sizeof(uint32_t) * (ng ? ng : 1));
+        for (uint32_t k = 0; k < ng; ++k) {
+            bl->geo_begin[k] = nt;
+            nt += geo_tris(&g[k]);
+        }
         bl->ntri = nt;
         bl->tri = (float *)malloc(sizeof(float) * 9 * (nt ? nt : 1));
         float (*lo)[3] = malloc(sizeof(float) * 3 * (nt ? nt : 1));
         float (*hi)[3] = malloc(sizeof(float) * 3 * (nt ? nt : 1));
+        uint32_t gk = 0;
         for (uint32_t t = 0; t < nt; ++t) {
+            while (gk + 1 < ng && bl->geo_begin[gk + 1] <= t) ++gk;
+            const sr_geometry *G = &g[gk];
+            const uint32_t lt = t - bl->geo_begin[gk];
             for (int k = 0; k < 3; ++k) {
-                uint32_t vi = g[b].indices[3 * t + k];
-                const float *p = (const float *)((const char *)g[b].vertices + (uint64_t)vi * g[b].vertex_stride);
+                uint32_t vi = G->indices ? G->indices[3 * lt + k] : 3 * lt + (uint32_t)k;
+                const float *p = (const float *)((const char *)G->vertices + (uint64_t)vi * G->vertex_stride);
                 bl->tri[9 * t + 3 * k + 0] = p[0];
                 bl->tri[9 * t + 3 * k + 1] = p[1];
                 bl->tri[9 * t + 3 * k + 2] = p[2];
@@ -508,6 +537,7 @@ sr_scene *sr_scene_create(const sr_geometry *g, uint32_t nblas,
         }
         build_bvh((const float (*)[3])lo, (const float (*)[3])hi, nt, 4, &bl->nodes, &bl->nnodes, &bl->order);
         free(lo); free(hi);
+        g += ng;
     }
     s->inst = (sr_instance_desc *)malloc(sizeof(sr_instance_desc) * (ninst ? ninst : 1));
     memcpy(s->inst, inst, sizeof(sr_instance_desc) * ninst);
@@ -541,14 +571,16 @@ void sr_scene_destroy(sr_scene *s)
 {
     if (!s) return;
     for (uint32_t b = 0; b < s->nblas; ++b) {
-        free(s->blas[b].tri); free(s->blas[b].nodes); free(s->blas[b].order);
+        free(s->blas[b].tri); free(s->blas[b].nodes); free(s->blas[b].order); free(s->blas[b].geo_begin);
     }
     free(s->blas); free(s->inst); free(s->inv); free(s->wlo); free(s->whi);
     free(s->tnodes); free(s->torder);
     free(s);
 }
 
-/* (t, inst, prim) lexicographic order: the documented tie rule */
+/* (t, inst, prim) lexicographic order with prim the BLAS-wide triangle
+ * index, i.e. (t, inst, geometry, triangle in geometry): the documented tie
+ * rule */
 static inline int better_hit(float t, uint32_t inst, uint32_t prim, const sr_hit *h)
 {
     if (t < h->t) return 1;
@@ -630,7 +662,7 @@ static void blas_closest(const blas_t *bl, const oray *r, float tmin, uint32_t i
 
 static void trace_one(const sr_scene *s, const sr_ray *ray, sr_hit *h, int use_bvh, uint32_t cull, int any)
 {
-    h->t = ray->tmax; h->u = 0.0f; h->v = 0.0f; h->inst = SR_MISS; h->prim = SR_MISS;
+    h->t = ray->tmax; h->u = 0.0f; h->v = 0.0f; h->inst = SR_MISS; h->prim = SR_MISS; h->geom = 0;
     const float o[3] = {ray->ox, ray->oy, ray->oz}, d[3] = {ray->dx, ray->dy, ray->dz};
     if (!use_bvh) {
         for (uint32_t i = 0; i < s->ninst; ++i) {
@@ -682,6 +714,19 @@ done:
     if (h->inst == SR_MISS) { h->t = -1.0f; h->u = 0.0f; h->v = 0.0f; }
 }
 
+/* a renderer-internal hit (BLAS-wide prim, geom 0) as a public record:
+ * (geometry, triangle in geometry), geom = SR_MISS on a miss */
+static sr_hit public_hit(const sr_scene *s, sr_hit h)
+{
+    if (h.inst == SR_MISS) { h.geom = SR_MISS; return h; }
+    const blas_t *bl = &s->blas[s->inst[h.inst].blas];
+    uint32_t gk = 0;
+    while (gk + 1 < bl->ngeo && bl->geo_begin[gk + 1] <= h.prim) ++gk;
+    h.geom = gk;
+    h.prim -= bl->geo_begin[gk];
+    return h;
+}
+
 void sr_trace(const sr_scene *s, const sr_ray *rays, sr_hit *hits, uint64_t n,
               int use_bvh, uint32_t cull_mask, int nthreads)
 {
@@ -689,8 +734,10 @@ void sr_trace(const sr_scene *s, const sr_ray *rays, sr_hit *hits, uint64_t n,
     if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
 #endif
-    for (int64_t i = 0; i < (int64_t)n; ++i)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
         trace_one(s, &rays[i], &hits[i], use_bvh, cull_mask, 0);
+        hits[i] = public_hit(s, hits[i]);
+    }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -848,7 +895,7 @@ void sr_render_ao(const sr_scene *s, const sr_tables *tb, const sr_uniforms *un,
             mk_ray(&ray, o, fmaxf(1.0f, length3(o)) * 1e-3f, d, 10000.0f);
             trace_one(s, &ray, &h, 1, 0xFFu, 0);
             rays += 1;
-            if (prim_hits) prim_hits[pix] = h;
+            if (prim_hits) prim_hits[pix] = public_hit(s, h);
             float color = 1.0f;
             if (h.inst != SR_MISS) {
                 surf_t sf;
@@ -868,14 +915,14 @@ void sr_render_ao(const sr_scene *s, const sr_tables *tb, const sr_uniforms *un,
                     mk_ray(&ar, sf.pos, p->ao_tmin, ad, p->ao_tmax);
                     trace_one(s, &ar, &ah, 1, 0xFFu, p->any_hit ? 1 : 0);
                     if (ah.inst != SR_MISS) occl++;
-                    if (ao_hits) ao_hits[pix * p->n_ao + k] = ah;
+                    if (ao_hits) ao_hits[pix * p->n_ao + k] = public_hit(s, ah);
                 }
                 rays += p->n_ao;
                 color = 1.0f - (float)occl / (float)p->n_ao;
             } else if (ao_hits) {
                 for (uint32_t k = 0; k < p->n_ao; ++k) {
                     sr_hit *m = &ao_hits[pix * p->n_ao + k];
-                    m->t = -1.0f; m->u = 0.0f; m->v = 0.0f; m->inst = SR_MISS; m->prim = SR_MISS;
+                    m->t = -1.0f; m->u = 0.0f; m->v = 0.0f; m->inst = SR_MISS; m->prim = SR_MISS; m->geom = SR_MISS;
                 }
             }
             float a = 1.0f / (float)(un->frame[2] - (uint32_t)accum_start + 1u);

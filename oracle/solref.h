@@ -76,9 +76,13 @@ typedef struct {
     uint32_t pad;
 } sr_uniforms;
 
-/* ray: 32 B {o.xyz, tmin, d.xyz, tmax};  hit: 20 B {t,u,v,inst,prim} */
+/* ray: 32 B {o.xyz, tmin, d.xyz, tmax};  hit: 24 B {t,u,v,inst,prim,geom}.
+ * Inside the renderers prim is the BLAS-wide triangle index (the geometries
+ * of a BLAS numbered one after another) and geom is 0; the hits sr_trace and
+ * sr_render_ao return carry gl_GeometryIndexEXT in geom and gl_PrimitiveID
+ * (the index within that geometry) in prim. */
 typedef struct { float ox, oy, oz, tmin, dx, dy, dz, tmax; } sr_ray;
-typedef struct { float t, u, v; uint32_t inst, prim; } sr_hit;
+typedef struct { float t, u, v; uint32_t inst, prim, geom; } sr_hit;
 #define SR_MISS 0xFFFFFFFFu
 
 /* ---- sampling.glsl / postprocess.glsl --------------------------------- */
@@ -107,19 +111,27 @@ typedef struct {
     const void *vertices;       /* first vertex of the geometry */
     uint64_t vertex_stride;     /* bytes; positions are 3 floats at offset 0 */
     uint32_t vertex_count;
-    const uint32_t *indices;    /* already offset to the geometry's first index */
+    const uint32_t *indices;    /* already offset to the geometry's first index;
+                                   NULL: non-indexed, triangle i = vertices 3i..3i+2
+                                   and vertex_count / 3 triangles
+                                   (src/ray/acceleration.rs:169-188) */
     uint32_t index_count;
 } sr_geometry;
 
 typedef struct sr_scene sr_scene;
-/* Builds the CPU acceleration structures.  Geometry memory must outlive the
- * scene. */
+/* Builds the CPU acceleration structures, one geometry per BLAS.  Geometry
+ * memory must outlive the scene. */
 sr_scene *sr_scene_create(const sr_geometry *blas, uint32_t nblas,
                           const sr_instance_desc *inst, uint32_t ninst);
+/* The same with geo_count[b] geometries in BLAS b (BLAS::new's
+ * Vec<GeometryInstance>, src/ray/acceleration.rs:136-208), the geometries of
+ * all BLASes back to back in geos. */
+sr_scene *sr_scene_create_multi(const sr_geometry *geos, const uint32_t *geo_count, uint32_t nblas,
+                                const sr_instance_desc *inst, uint32_t ninst);
 void      sr_scene_destroy(sr_scene *s);
 
-/* Closest hit for n rays: (t, instance, primitive) lexicographic minimum over
- * all hits with t in [tmin, tmax].  use_bvh = 0: brute force over every
+/* Closest hit for n rays: (t, instance, geometry, primitive) lexicographic
+ * minimum over all hits with t in [tmin, tmax].  use_bvh = 0: brute force over every
  * triangle; 1: through the oracle's binary BVH (same answer).  cull_mask is
  * ANDed with the instance mask.  Threads: OpenMP, nthreads <= 0 = all. */
 void sr_trace(const sr_scene *s, const sr_ray *rays, sr_hit *hits, uint64_t n,

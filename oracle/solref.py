@@ -12,7 +12,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsolref.so")
 
 RAY_DTYPE = np.dtype([("o", "<f4", 3), ("tmin", "<f4"), ("d", "<f4", 3), ("tmax", "<f4")])
-HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("inst", "<u4"), ("prim", "<u4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("inst", "<u4"), ("prim", "<u4"),
+                      ("geom", "<u4")])
 MISS = 0xFFFFFFFF
 
 
@@ -77,6 +78,8 @@ def lib():
         L.sr_intersect_triangle.argtypes = [vp, vp, f32, f32, vp, vp, vp, vp, vp, vp]
         L.sr_scene_create.restype = vp
         L.sr_scene_create.argtypes = [vp, u32, vp, u32]
+        L.sr_scene_create_multi.restype = vp
+        L.sr_scene_create_multi.argtypes = [vp, vp, u32, vp, u32]
         L.sr_scene_destroy.argtypes = [vp]
         L.sr_trace.argtypes = [vp, vp, vp, u64, C.c_int, u32, C.c_int]
         L.sr_render_ao_glsl.argtypes = [vp, vp, vp, i32, vp, vp, u32, u32, u32, u32, u32, u32, C.c_int]
@@ -186,19 +189,32 @@ class Scene:
     def __init__(self, sd):
         L = lib()
         self._keep = []
-        geos = (Geometry * max(1, len(sd.blas)))()
-        for i, b in enumerate(sd.blas):
-            v = np.ascontiguousarray(b.vertices)
-            idx = np.ascontiguousarray(b.indices, np.uint32)
-            self._keep += [v, idx]
+        # every BLAS as its list of geometries (vertices, indices or None):
+        # a scenes.Blas is one indexed geometry
+        glist, counts = [], []
+        for b in sd.blas:
+            gs = getattr(b, "geometries", None) or [(b.vertices, b.indices)]
+            glist += gs
+            counts.append(len(gs))
+        geos = (Geometry * max(1, len(glist)))()
+        for i, (verts, inds) in enumerate(glist):
+            v = np.ascontiguousarray(verts)
+            self._keep.append(v)
             geos[i].vertices = v.ctypes.data
             geos[i].vertex_stride = v.dtype.itemsize
             geos[i].vertex_count = len(v)
-            geos[i].indices = idx.ctypes.data
-            geos[i].index_count = len(idx)
+            if inds is None:
+                geos[i].indices = None
+                geos[i].index_count = 0
+            else:
+                idx = np.ascontiguousarray(inds, np.uint32)
+                self._keep.append(idx)
+                geos[i].indices = idx.ctypes.data
+                geos[i].index_count = len(idx)
+        cnt = np.ascontiguousarray(counts if counts else [0], np.uint32)
         inst = np.ascontiguousarray(sd.instance_descs)
-        self._keep.append(inst)
-        self.h = L.sr_scene_create(C.cast(geos, C.c_void_p), len(sd.blas), _p(inst), len(inst))
+        self._keep += [inst, cnt]
+        self.h = L.sr_scene_create_multi(C.cast(geos, C.c_void_p), _p(cnt), len(sd.blas), _p(inst), len(inst))
         self._geos = geos
         # shading tables: per-object vertex / index pointers, per-instance materials
         self.sd = sd

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -78,6 +79,29 @@ hipStream_t stream_of(Context *cx, solhip_stream s, bool &ok)
     return st->s;
 }
 
+uint64_t next_generation()
+{
+    static std::atomic<uint64_t> g{0};
+    return ++g;
+}
+
+int check_fresh(const Tlas *t)
+{
+    if (!t->built) {
+        set_error("the TLAS's last build failed; rebuild it before tracing");
+        return SOLHIP_ERROR_STALE;
+    }
+    for (size_t k = 0; k < t->blas_handles.size(); ++k) {
+        const Blas *b = get<Blas>(t->blas_handles[k], Kind::Blas);
+        if (!b || b->gen != t->blas_gens[k]) {
+            set_error(std::string("a BLAS of this TLAS was ") + (b ? "rebuilt" : "destroyed") +
+                      " after the TLAS was built; rebuild the TLAS (solhip_tlas_rebuild) before tracing");
+            return SOLHIP_ERROR_STALE;
+        }
+    }
+    return SOLHIP_OK;
+}
+
 // LDS stack entries a traversal of this TLAS needs: one per TLAS level plus
 // one per level of the deepest referenced BLAS (plus one spare)
 uint32_t stack_depth_of(const Tlas *t)
@@ -111,14 +135,23 @@ int tlas_build_into(Context *cx, hipStream_t s, Tlas *t, const solhip_instance *
 {
     std::vector<BlasRef> refs(n);
     std::vector<uint32_t> tris(n ? n : 1, 0u);
+    t->built = false;
+    t->multi_geo = false;
     t->blas_handles.clear();
+    t->blas_gens.clear();
     for (uint32_t i = 0; i < n; ++i) {
         Blas *b = get<Blas>(inst[i].blas, Kind::Blas);
-        if (b && std::find(t->blas_handles.begin(), t->blas_handles.end(), inst[i].blas) == t->blas_handles.end())
-            t->blas_handles.push_back(inst[i].blas);
         if (!b) {
             set_error("instance " + std::to_string(i) + " references an invalid BLAS handle");
             return SOLHIP_ERROR_INVALID_HANDLE;
+        }
+        if (b->gen == 0) {
+            set_error("instance " + std::to_string(i) + " references a BLAS whose last build failed");
+            return SOLHIP_ERROR_STALE;
+        }
+        if (std::find(t->blas_handles.begin(), t->blas_handles.end(), inst[i].blas) == t->blas_handles.end()) {
+            t->blas_handles.push_back(inst[i].blas);
+            t->blas_gens.push_back(b->gen);
         }
         if (b->bvh.prim_count == 0) {
             refs[i].nodes = nullptr;
@@ -128,7 +161,8 @@ int tlas_build_into(Context *cx, hipStream_t s, Tlas *t, const solhip_instance *
             refs[i].tris = (const TriRec *)b->bvh.leaves.ptr;
         }
         refs[i].root_box = (const float *)b->bvh.root_box.ptr;
-        refs[i].pad = 0;
+        refs[i].geo = b->geo_table();
+        if (refs[i].geo) t->multi_geo = true;
         tris[i] = b->bvh.prim_count;
     }
     const uint64_t nn = n ? n : 1;
@@ -143,8 +177,82 @@ int tlas_build_into(Context *cx, hipStream_t s, Tlas *t, const solhip_instance *
         SOLHIP_HIP_CHECK(hipMemcpy(t->refs_dev.ptr, refs.data(), (uint64_t)n * sizeof(BlasRef), hipMemcpyHostToDevice));
     }
     (void)cx;
-    return build_tlas(s, (const solhip_instance *)t->inst_dev.ptr, (const BlasRef *)t->refs_dev.ptr, n, t->scratch,
-                      t->bvh);
+    const int rc = build_tlas(s, (const solhip_instance *)t->inst_dev.ptr, (const BlasRef *)t->refs_dev.ptr, n,
+                              t->scratch, t->bvh);
+    if (rc) return rc;
+    t->built = true;
+    t->gen = next_generation();
+    return SOLHIP_OK;
+}
+
+// Checks the geometries of a BLAS build (BLAS::new's rules, see
+// solhip_geometry) and fills the build kernels' descriptors; *tris = the
+// BLAS-wide triangle count.
+static int geometry_descs(const solhip_geometry *g, uint32_t n, std::vector<GeomDesc> &out, uint32_t *tris)
+{
+    if (!g || n == 0) return bad_arg("at least one geometry");
+    if (n > (1u << 24)) return bad_arg("more than 2^24 geometries");
+    out.resize(n);
+    uint64_t total = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const solhip_geometry &G = g[k];
+        const std::string at = "geometry " + std::to_string(k) + ": ";
+        if (G.vertex_stride < 12 || G.vertex_stride % 4) return bad_arg(at + "vertex_stride must be >= 12 and a multiple of 4");
+        uint64_t prims;
+        GeomDesc &d = out[k];
+        d.stride = G.vertex_stride;
+        d.first_vertex = G.first_vertex;
+        d.vertex_count = G.vertex_count;
+        if (G.index_addr) {
+            if (G.index_count % 3) return bad_arg(at + "index_count must be a multiple of 3");
+            if (G.primitive_offset % 4) return bad_arg(at + "primitive_offset must be a multiple of 4 (UINT32 indices)");
+            prims = G.index_count / 3;
+            if (prims && (!G.vertex_addr || G.vertex_count == 0))
+                return bad_arg(at + "indexed triangles need a vertex address and vertex_count > 0");
+            d.vbase = reinterpret_cast<const uint8_t *>(G.vertex_addr);
+            d.idx = reinterpret_cast<const uint32_t *>(G.index_addr + G.primitive_offset);
+        } else {
+            if (G.index_count) return bad_arg(at + "index_count > 0 without an index address");
+            prims = G.vertex_count / 3;
+            if (prims && !G.vertex_addr) return bad_arg(at + "null vertex address");
+            d.vbase = reinterpret_cast<const uint8_t *>(G.vertex_addr + G.primitive_offset);
+            d.idx = nullptr;
+        }
+        if ((uint64_t)G.first_vertex + G.vertex_count > 0xFFFFFFFFull) return bad_arg(at + "first_vertex + vertex_count >= 2^32");
+        d.tri_begin = (uint32_t)total;
+        d.tri_count = (uint32_t)prims;
+        total += prims;
+        if (total > 0x7FFFFFFFull) return bad_arg("more than 2^31 - 1 triangles in one BLAS");
+    }
+    *tris = (uint32_t)total;
+    return SOLHIP_OK;
+}
+
+// (re)builds b over the geometries: descriptors to the device, then the BVH
+static int blas_build_into(hipStream_t s, Blas *b, const solhip_geometry *g, uint32_t n)
+{
+    std::vector<GeomDesc> descs;
+    uint32_t tris = 0;
+    int rc = geometry_descs(g, n, descs, &tris);
+    if (rc) return rc;
+    b->gen = 0;   // stale until the build succeeds
+    // descriptors, then the u32 table [n, tri_begin...] that TLAS records
+    // point to when there are several geometries
+    const uint64_t dbytes = (uint64_t)n * sizeof(GeomDesc);
+    std::vector<uint8_t> blob(dbytes + 4ull * (n + 1));
+    std::memcpy(blob.data(), descs.data(), dbytes);
+    uint32_t *tab = reinterpret_cast<uint32_t *>(blob.data() + dbytes);
+    tab[0] = n;
+    for (uint32_t k = 0; k < n; ++k) tab[1 + k] = descs[k].tri_begin;
+    if (b->geo_dev.reserve(blob.size())) return SOLHIP_ERROR_OUT_OF_MEMORY;
+    // the previous build's kernels may still read the table
+    SOLHIP_HIP_CHECK(hipStreamSynchronize(s));
+    SOLHIP_HIP_CHECK(hipMemcpy(b->geo_dev.ptr, blob.data(), blob.size(), hipMemcpyHostToDevice));
+    b->n_geo = n;
+    rc = build_blas(s, (const GeomDesc *)b->geo_dev.ptr, n, tris, b->build_flags, b->scratch, b->bvh);
+    if (rc) return rc;
+    b->gen = next_generation();
+    return SOLHIP_OK;
 }
 
 // emitter table: counts and offsets on the device, one read-back of the
@@ -303,7 +411,7 @@ int trace_wavefront(Context *c, hipStream_t s, Program *p, Tlas *t, const ProgAr
     nslices = std::max(1u, std::min(nslices, tiles32));
     if (p->st0.reserve(np * 16) || p->st1.reserve(np * 16) || p->psum.reserve(npix * 16) ||
         (w.streams && p->fsum.reserve(npix * 32)) ||
-        p->q0.reserve(ns * sizeof(QRay)) || p->q1.reserve(ns * sizeof(QRay)) || p->hits.reserve(ns * sizeof(solhip_hit)) ||
+        p->q0.reserve(ns * sizeof(QRay)) || p->q1.reserve(ns * sizeof(QRay)) || p->hits.reserve(ns * sizeof(QHit)) ||
         p->sq.reserve(ns * sizeof(QRay)) || p->scontrib.reserve(ns * 16) || p->counts.reserve(256ull * nslices) ||
         p->heads.reserve(nheads * wf_head_words() * 4 * nslices) || p->fc.reserve(sizeof(FrameConsts)))
         return SOLHIP_ERROR_OUT_OF_MEMORY;
@@ -319,7 +427,7 @@ int trace_wavefront(Context *c, hipStream_t s, Program *p, Tlas *t, const ProgAr
     w.fsum = w.streams ? (FixSum *)p->fsum.ptr : nullptr;
     w.q[0] = (QRay *)p->q0.ptr;
     w.q[1] = (QRay *)p->q1.ptr;
-    w.hits = (solhip_hit *)p->hits.ptr;
+    w.hits = (QHit *)p->hits.ptr;
     w.sq = (QRay *)p->sq.ptr;
     w.scontrib = (float4 *)p->scontrib.ptr;
     w.counts = (uint32_t *)p->counts.ptr;
@@ -484,7 +592,11 @@ int32_t solhip_context_destroy(solhip_ctx ctx)
         switch (o->kind) {
         case Kind::Stream: (void)hipStreamDestroy(static_cast<Stream *>(o)->s); break;
         case Kind::Buffer: (void)hipFree(static_cast<Buffer *>(o)->ptr); break;
-        case Kind::Blas: static_cast<Blas *>(o)->bvh.release(); static_cast<Blas *>(o)->scratch.release(); break;
+        case Kind::Blas:
+            static_cast<Blas *>(o)->bvh.release();
+            static_cast<Blas *>(o)->scratch.release();
+            static_cast<Blas *>(o)->geo_dev.release();
+            break;
         case Kind::Tlas: {
             Tlas *t = static_cast<Tlas *>(o);
             t->bvh.release();
@@ -634,20 +746,16 @@ int32_t solhip_blas_build(solhip_ctx ctx, solhip_stream st, const solhip_geometr
     if (!ok) return bad_handle("stream");
     if (!g || !out) return bad_arg("geometry / out");
     if (flags & ~(uint32_t)SOLHIP_BUILD_PREFER_FAST_TRACE) return bad_arg("unknown build flags");
-    if (n != 1) {
-        set_error("exactly one geometry per BLAS is supported (SceneDescription builds one BLAS per primitive)");
-        return SOLHIP_ERROR_UNSUPPORTED;
-    }
-    if (g->index_count && (!g->vertex_addr || !g->index_addr)) return bad_arg("null vertex / index address");
-    if (g->vertex_stride < 12) return bad_arg("vertex_stride < 12");
     Blas *b = new Blas();
     b->kind = Kind::Blas;
     b->ctx = ctx;
     b->build_flags = flags;
-    int rc = build_blas(s, *g, flags, b->scratch, b->bvh);
+    int rc = blas_build_into(s, b, g, n);
     if (rc) {
+        (void)hipStreamSynchronize(s);
         b->bvh.release();
         b->scratch.release();
+        b->geo_dev.release();
         delete b;
         return rc;
     }
@@ -664,8 +772,7 @@ int32_t solhip_blas_rebuild(solhip_ctx ctx, solhip_stream st, solhip_blas bh, co
     if (!ok) return bad_handle("stream");
     Blas *b = get<Blas>(bh, Kind::Blas);
     if (!b) return bad_handle("BLAS");
-    if (!g || n != 1) return bad_arg("exactly one geometry");
-    return build_blas(s, *g, b->build_flags, b->scratch, b->bvh);
+    return blas_build_into(s, b, g, n);
 }
 
 int32_t solhip_blas_destroy(solhip_ctx ctx, solhip_blas bh)
@@ -677,6 +784,7 @@ int32_t solhip_blas_destroy(solhip_ctx ctx, solhip_blas bh)
     (void)hipDeviceSynchronize();
     b->bvh.release();
     b->scratch.release();
+    b->geo_dev.release();
     delete b;
     return SOLHIP_OK;
 }
@@ -782,12 +890,15 @@ int32_t solhip_trace_rays(solhip_ctx ctx, solhip_stream st, solhip_tlas th, solh
     if (!rb || !hb) return bad_handle("buffer");
     if ((uint64_t)n * sizeof(solhip_ray) > rb->bytes) return bad_arg("ray buffer smaller than n rays");
     if ((uint64_t)n * sizeof(solhip_hit) > hb->bytes) return bad_arg("hit buffer smaller than n hits");
+    if (const int rc = check_fresh(t)) return rc;
     if (stack_depth_of(t) > (uint32_t)kMaxStack) {
         set_error("acceleration structures too deep for the traversal stack");
         return SOLHIP_ERROR_UNSUPPORTED;
     }
-    return launch_trace_rays(s, view_of(t), (const solhip_ray *)rb->ptr, (solhip_hit *)hb->ptr, n, flags,
-                             c->counters + SOLHIP_KERNEL_TRACE_RAYS, c->timer_or_null());
+    const int rc = launch_trace_rays(s, view_of(t), (const solhip_ray *)rb->ptr, (solhip_hit *)hb->ptr, n, flags,
+                                     c->counters + SOLHIP_KERNEL_TRACE_RAYS, c->timer_or_null());
+    if (rc || !t->multi_geo) return rc;
+    return launch_split_geometry(s, (solhip_hit *)hb->ptr, n, (const BlasRef *)t->refs_dev.ptr, t->n_inst);
 }
 
 void solhip_spec_default(uint32_t kind, solhip_spec *spec)
@@ -864,6 +975,17 @@ int32_t solhip_trace(solhip_ctx ctx, solhip_stream st, solhip_program ph, solhip
                      const solhip_scene_tables *tables, const solhip_scene_uniforms *u, int32_t accum_start_frame,
                      const solhip_images *images, uint32_t W, uint32_t H)
 {
+    return trace_program(ctx, st, ph, th, tables, u, accum_start_frame, images, W, H, nullptr);
+}
+
+}  // extern "C"
+
+namespace solhip {
+
+int trace_program(solhip_ctx ctx, solhip_stream st, solhip_program ph, solhip_tlas th,
+                  const solhip_scene_tables *tables, const solhip_scene_uniforms *u, int32_t accum_start_frame,
+                  const solhip_images *images, uint32_t W, uint32_t H, const FixedScratch *fixed)
+{
     Context *c = ctx_of(ctx);
     if (!c) return bad_handle("context");
     bool ok;
@@ -873,6 +995,12 @@ int32_t solhip_trace(solhip_ctx ctx, solhip_stream st, solhip_program ph, solhip
     if (!p) return bad_handle("program");
     Tlas *t = get<Tlas>(th, Kind::Tlas);
     if (!t) return bad_handle("TLAS");
+    if (fixed && p->kind_id == SOLHIP_PROG_PATHTRACE) {
+        set_error("the wavefront path tracer sizes its path state per image: no fixed-scratch variant");
+        return SOLHIP_ERROR_UNSUPPORTED;
+    }
+    // the fixed-scratch variant records no timing events (they would be allocations)
+    Timer *timer = fixed ? nullptr : c->timer_or_null();
     if (!u || !images) return bad_arg("uniforms / images");
     if (W == 0 || H == 0) return SOLHIP_OK;
     Buffer *acc = get<Buffer>(images->accum, Kind::Buffer);
@@ -888,6 +1016,11 @@ int32_t solhip_trace(solhip_ctx ctx, solhip_stream st, solhip_program ph, solhip
             !tables->material_table_addr)
             return bad_arg("pathtrace needs the material table");
     }
+    if (const int rc = check_fresh(t)) return rc;
+    if (p->kind_id == SOLHIP_PROG_PATHTRACE && p->spec.sample_streams &&
+        (uint64_t)p->spec.spp_per_frame * npix > (1ull << 32))
+        return bad_arg("sample_streams: spp_per_frame * W * H must not exceed 2^32 (sample s of pixel p seeds "
+                       "tea(p + s * W * H, frame) in 32 bits)");
     if (stack_depth_of(t) > (uint32_t)kMaxStack) {
         set_error("acceleration structures too deep for the traversal stack");
         return SOLHIP_ERROR_UNSUPPORTED;
@@ -947,25 +1080,59 @@ int32_t solhip_trace(solhip_ctx ctx, solhip_stream st, solhip_program ph, solhip
     switch (p->kind_id) {
     case SOLHIP_PROG_AO: {
         const uint64_t hb = ao_head_bytes();
-        if (p->queue.reserve(npix * sizeof(AoEntry)) || p->qcount.reserve(256) || p->occl.reserve(npix * a.n_ao) ||
-            (hb && p->aoheads.reserve(hb)))
-            return SOLHIP_ERROR_OUT_OF_MEMORY;
-        a.aoheads = hb ? (uint32_t *)p->aoheads.ptr : nullptr;
-        a.queue = (AoEntry *)p->queue.ptr;
-        a.qcount = (uint32_t *)p->qcount.ptr;
-        a.occl = (uint8_t *)p->occl.ptr;
+        if (fixed) {
+            const AoScratchLayout L = ao_scratch_layout(npix, a.n_ao);
+            if (L.total > fixed->bytes) {
+                set_error("W*H*(32 + n_ao) + " + std::to_string(L.total - npix * (32 + a.n_ao)) + " = " +
+                          std::to_string(L.total) + " bytes exceed the fixed scratch of " +
+                          std::to_string(fixed->bytes) + " bytes");
+                return SOLHIP_ERROR_INVALID_ARGUMENT;
+            }
+            uint8_t *base = (uint8_t *)fixed->ptr;
+            a.qcount = (uint32_t *)(base + L.qcount);
+            a.aoheads = hb ? (uint32_t *)(base + L.heads) : nullptr;
+            a.queue = (AoEntry *)(base + L.queue);
+            a.occl = base + L.occl;
+        } else {
+            if (p->queue.reserve(npix * sizeof(AoEntry)) || p->qcount.reserve(256) ||
+                p->occl.reserve(npix * a.n_ao) || (hb && p->aoheads.reserve(hb)))
+                return SOLHIP_ERROR_OUT_OF_MEMORY;
+            a.aoheads = hb ? (uint32_t *)p->aoheads.ptr : nullptr;
+            a.queue = (AoEntry *)p->queue.ptr;
+            a.qcount = (uint32_t *)p->qcount.ptr;
+            a.occl = (uint8_t *)p->occl.ptr;
+        }
         a.ncu = c->ncu;
         int rc = launch_fill(s, a.qcount, 2, 0u);
         if (rc) return rc;
-        return launch_ao(s, a, c->timer_or_null());
+        rc = launch_ao(s, a, timer);
+        if (rc || !a.hits_out || !t->multi_geo) return rc;
+        return launch_split_geometry(s, a.hits_out, npix * (1 + a.n_ao), (const BlasRef *)t->refs_dev.ptr,
+                                     t->n_inst);
     }
-    case SOLHIP_PROG_AO_GLSL: return launch_ao_glsl(s, a, c->timer_or_null());
-    case SOLHIP_PROG_PATHTRACE_GLSL: return launch_pathtrace_glsl(s, a, c->timer_or_null());
-    case SOLHIP_PROG_DEBUG: return launch_debug(s, a, c->timer_or_null());
+    case SOLHIP_PROG_AO_GLSL: return launch_ao_glsl(s, a, timer);
+    case SOLHIP_PROG_PATHTRACE_GLSL: return launch_pathtrace_glsl(s, a, timer);
+    case SOLHIP_PROG_DEBUG: return launch_debug(s, a, timer);
     case SOLHIP_PROG_PATHTRACE: return trace_wavefront(c, s, p, t, a);
     default: return bad_arg("program kind");
     }
 }
+
+AoScratchLayout ao_scratch_layout(uint64_t npix, uint32_t n_ao)
+{
+    auto up = [](uint64_t x) { return (x + 255) & ~255ull; };
+    AoScratchLayout L;
+    L.qcount = 0;
+    L.heads = 256;
+    L.queue = L.heads + up(ao_head_bytes());
+    L.occl = L.queue + npix * sizeof(AoEntry);
+    L.total = L.occl + npix * n_ao;
+    return L;
+}
+
+}  // namespace solhip
+
+extern "C" {
 
 int32_t solhip_pathtrace_autotune(solhip_ctx ctx, solhip_stream st, solhip_tlas th, const solhip_scene_tables *tables,
                                   const solhip_scene_uniforms *u, const solhip_images *images, uint32_t W, uint32_t H,
@@ -980,6 +1147,7 @@ int32_t solhip_pathtrace_autotune(solhip_ctx ctx, solhip_stream st, solhip_tlas 
     Tlas *t = get<Tlas>(th, Kind::Tlas);
     if (!t) return bad_handle("TLAS");
     if (!spec || !best || !u || !images || frames == 0) return bad_arg("spec / best / uniforms / images / frames");
+    if (const int rc = check_fresh(t)) return rc;
     // workload key: device, CUs, image, estimator settings, scene size
     uint64_t tris = 0;
     for (uint64_t h : t->blas_handles) {

@@ -6,6 +6,7 @@
 
 #include <vector>
 
+#include "bvh_logic.h"
 #include "common.h"
 
 namespace solhip {
@@ -18,13 +19,17 @@ struct DevBuf {
     void release();
 };
 
-// per-instance reference to a BLAS, read by the TLAS build kernels
+// per-instance reference to a BLAS, read by the TLAS build kernels and by
+// the pass that splits BLAS-wide triangle indices of hits into (geometry,
+// triangle)
 struct BlasRef {
     const WideNode *nodes;   // nullptr: empty BLAS
     const TriRec *tris;
     const float *root_box;   // 6 floats (lo xyz, hi xyz) in device memory
-    uint64_t pad;
+    const uint32_t *geo;     // [n_geo, first triangle of geometry 0 .. n_geo-1], or nullptr (one geometry)
 };
+
+
 static_assert(sizeof(BlasRef) == 32, "BlasRef layout");
 
 // build scratch, kept by each BLAS / TLAS so that rebuilds allocate nothing
@@ -54,8 +59,13 @@ struct DeviceBvh {
     }
 };
 
-// flags: SOLHIP_BUILD_* (SOLHIP_BUILD_PREFER_FAST_TRACE: binned SAH, else Morton order)
-int build_blas(hipStream_t s, const solhip_geometry &g, uint32_t flags, DevScratch &sc, DeviceBvh &out);
+// flags: SOLHIP_BUILD_* (SOLHIP_BUILD_PREFER_FAST_TRACE: binned SAH, else Morton order).
+// geo_dev: the n_geo descriptors (and, after them, the u32 table
+// [n_geo, tri_begin...] that BlasRef::geo points to) in device memory; n
+// triangles in all.  An index >= vertex_count fails the build with
+// SOLHIP_ERROR_INVALID_ARGUMENT.
+int build_blas(hipStream_t s, const GeomDesc *geo_dev, uint32_t n_geo, uint32_t n, uint32_t flags, DevScratch &sc,
+               DeviceBvh &out);
 int build_tlas(hipStream_t s, const solhip_instance *inst_dev, const BlasRef *refs_dev, uint32_t n, DevScratch &sc,
                DeviceBvh &out);
 

@@ -80,26 +80,31 @@ __device__ void reduce_bounds(float c[3], bool valid, uint32_t *bounds)
     }
 }
 
-// vertex of index value `iv`; out-of-range index values are clamped to the
-// last vertex so that malformed input cannot read outside the buffer
-__device__ __forceinline__ const float *vtx(const uint8_t *vbase, uint64_t stride, uint32_t first_vertex,
-                                            uint32_t vertex_count, uint32_t iv)
+__device__ __forceinline__ const GeomDesc &geom_of(const GeomDesc *geo, uint32_t n_geo, uint32_t t)
 {
-    const uint32_t r = iv < vertex_count ? iv : (vertex_count ? vertex_count - 1 : 0);
-    return reinterpret_cast<const float *>(vbase + (uint64_t)(first_vertex + r) * stride);
+    return geo[geom_index(geo, n_geo, t)];
 }
 
-__global__ void __launch_bounds__(kBlock) k_tri_boxes(const uint8_t *vbase, uint64_t stride, uint32_t first_vertex,
-                                                      uint32_t vertex_count, const uint32_t *idx, uint32_t n,
-                                                      float4 *boxes, uint32_t *bounds)
+// vertex k of BLAS-wide triangle t of geometry G; an index value >=
+// vertex_count is counted in *bad (the build then fails with an error)
+__device__ __forceinline__ const float *tri_vertex(const GeomDesc &G, uint32_t t, int k, uint32_t *bad)
+{
+    uint32_t iv;
+    if (!tri_vertex_index(G, t, k, iv)) atomicAdd(bad, 1u);
+    return reinterpret_cast<const float *>(G.vbase + (uint64_t)(G.first_vertex + iv) * G.stride);
+}
+
+__global__ void __launch_bounds__(kBlock) k_tri_boxes(const GeomDesc *geo, uint32_t n_geo, uint32_t n, float4 *boxes,
+                                                      uint32_t *bounds, uint32_t *bad)
 {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     float c[3] = {0.0f, 0.0f, 0.0f};
     const bool valid = i < n;
     if (valid) {
+        const GeomDesc &G = geom_of(geo, n_geo, i);
         float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
         for (int k = 0; k < 3; ++k) {
-            const float *p = vtx(vbase, stride, first_vertex, vertex_count, idx[3ull * i + k]);
+            const float *p = tri_vertex(G, i, k, bad);
             for (int a = 0; a < 3; ++a) {
                 lo[a] = fminf(lo[a], p[a]);
                 hi[a] = fmaxf(hi[a], p[a]);
@@ -672,22 +677,23 @@ __global__ void __launch_bounds__(64) k_collapse(CollapseArgs A)
     if (collapse_node(A, w, DevAlloc{A.counters})) atomicAdd(&A.counters[2], 1u);
 }
 
-__global__ void __launch_bounds__(kBlock) k_write_tris(const uint32_t *leaf_ids, uint32_t n, const uint8_t *vbase,
-                                                       uint64_t stride, uint32_t first_vertex, uint32_t vertex_count,
-                                                       const uint32_t *idx, TriRec *out)
+__global__ void __launch_bounds__(kBlock) k_write_tris(const uint32_t *leaf_ids, uint32_t n, const GeomDesc *geo,
+                                                       uint32_t n_geo, TriRec *out, uint32_t *bad)
 {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const uint32_t p = leaf_ids[i];
+    const GeomDesc &G = geom_of(geo, n_geo, p);
     TriRec r;
     for (int k = 0; k < 3; ++k) {
-        const float *q = vtx(vbase, stride, first_vertex, vertex_count, idx[3ull * p + k]);
+        const float *q = tri_vertex(G, p, k, bad);
         r.v[3 * k] = q[0];
         r.v[3 * k + 1] = q[1];
         r.v[3 * k + 2] = q[2];
     }
     r.prim = p;
-    r.pad[0] = r.pad[1] = 0;
+    r.geom = (uint32_t)(&G - geo);
+    r.lprim = p - G.tri_begin;
     out[i] = r;
 }
 
@@ -879,15 +885,13 @@ static int build_binary_sah(hipStream_t s, uint32_t n, DevScratch &sc, const uin
 // order and radix tree, or binned SAH when `sah`), then the 8-wide collapse.
 static int build_common(hipStream_t s, uint32_t n, uint32_t leaf_max, bool sah, DevScratch &sc, DeviceBvh &out)
 {
-    // counters: [0] wide nodes allocated, [1] leaf records allocated,
-    // [2] collapse range errors, [3] sort / refit / SAH range errors, [4] invalid
-    // nodes, [5] refit frontier size, [6] SAH tasks of the next level, [7] SAH
-    // internal nodes allocated
+    // counters (set by reset_build_counters before the box kernel): [0] wide
+    // nodes allocated, [1] leaf records allocated, [2] collapse range errors,
+    // [3] sort / refit / SAH range errors, [4] invalid nodes, [5] refit
+    // frontier size, [6] SAH tasks of the next level, [7] SAH internal nodes
+    // allocated, [8] SAH big tasks, [9] vertex references out of range
     uint32_t *counters = (uint32_t *)sc.counters.ptr;
     uint32_t *pinned = sc.pinned;
-    pinned[8] = 1u;
-    pinned[9] = pinned[10] = pinned[11] = pinned[12] = pinned[13] = 0u;
-    SOLHIP_HIP_CHECK(hipMemcpyAsync(counters, pinned + 8, 24, hipMemcpyHostToDevice, s));
     uint32_t *left = (uint32_t *)sc.left.ptr, *right = (uint32_t *)sc.right.ptr;
     uint32_t *rfirst = (uint32_t *)sc.rfirst.ptr;
     float4 *nbox = (float4 *)sc.nbox.ptr;
@@ -926,8 +930,13 @@ static int build_common(hipStream_t s, uint32_t n, uint32_t leaf_max, bool sah, 
         a.begin = begin;
         a.end = end;
         hipLaunchKernelGGL(k_collapse, dim3(cdiv(end - begin, 64)), dim3(64), 0, s, a);
-        SOLHIP_HIP_CHECK(hipMemcpyAsync(pinned, counters, 16, hipMemcpyDeviceToHost, s));
+        SOLHIP_HIP_CHECK(hipMemcpyAsync(pinned, counters, 40, hipMemcpyDeviceToHost, s));
         SOLHIP_HIP_CHECK(hipStreamSynchronize(s));
+        if (pinned[9]) {
+            set_error("BLAS geometry holds " + std::to_string(pinned[9]) +
+                      " vertex references at or beyond its vertex_count");
+            return SOLHIP_ERROR_INVALID_ARGUMENT;
+        }
         if (pinned[2] || pinned[3]) {
             set_error("BVH build range check failed (collapse " + std::to_string(pinned[2]) + ", sort/refit " +
                       std::to_string(pinned[3]) + ")");
@@ -1050,13 +1059,21 @@ void DevScratch::release()
 
 static const uint32_t kOrdInit[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
 
-int build_blas(hipStream_t s, const solhip_geometry &g, uint32_t flags, DevScratch &sc, DeviceBvh &out)
+// build counters before the first kernel of a build: [0] wide nodes
+// allocated (the root), [1]-[5] 0, [6]-[8] SAH task counts (0), [9] vertex
+// references out of range (0); see build_common
+static int reset_build_counters(hipStream_t s, DevScratch &sc)
 {
-    if (g.index_count % 3 != 0) {
-        set_error("index_count must be a multiple of 3");
-        return SOLHIP_ERROR_INVALID_ARGUMENT;
-    }
-    const uint32_t n = g.index_count / 3;
+    uint32_t *pinned = sc.pinned;
+    pinned[16] = 1u;
+    for (int k = 17; k < 26; ++k) pinned[k] = 0u;
+    SOLHIP_HIP_CHECK(hipMemcpyAsync(sc.counters.ptr, pinned + 16, 40, hipMemcpyHostToDevice, s));
+    return 0;
+}
+
+int build_blas(hipStream_t s, const GeomDesc *geo_dev, uint32_t n_geo, uint32_t n, uint32_t flags, DevScratch &sc,
+               DeviceBvh &out)
+{
     int rc = sc.ensure(n);
     if (rc) return rc;
     const uint64_t nn = std::max<uint32_t>(n, 1);
@@ -1072,15 +1089,16 @@ int build_blas(hipStream_t s, const solhip_geometry &g, uint32_t flags, DevScrat
         out.depth = 0;
         return 0;
     }
+    rc = reset_build_counters(s, sc);
+    if (rc) return rc;
+    uint32_t *bad = (uint32_t *)sc.counters.ptr + 9;
     SOLHIP_HIP_CHECK(hipMemcpyAsync(sc.bounds.ptr, kOrdInit, 24, hipMemcpyHostToDevice, s));
-    const uint8_t *vbase = reinterpret_cast<const uint8_t *>(g.vertex_addr);
-    const uint32_t *idx = reinterpret_cast<const uint32_t *>(g.index_addr + g.index_byte_offset);
-    hipLaunchKernelGGL(k_tri_boxes, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, vbase, g.vertex_stride, g.first_vertex,
-                       g.vertex_count, idx, n, (float4 *)sc.boxes.ptr, (uint32_t *)sc.bounds.ptr);
+    hipLaunchKernelGGL(k_tri_boxes, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, geo_dev, n_geo, n,
+                       (float4 *)sc.boxes.ptr, (uint32_t *)sc.bounds.ptr, bad);
     rc = build_common(s, n, kLeafMaxTri, (flags & SOLHIP_BUILD_PREFER_FAST_TRACE) != 0, sc, out);
     if (rc) return rc;
     hipLaunchKernelGGL(k_write_tris, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, (const uint32_t *)sc.leaf_ids.ptr, n,
-                       vbase, g.vertex_stride, g.first_vertex, g.vertex_count, idx, (TriRec *)out.leaves.ptr);
+                       geo_dev, n_geo, (TriRec *)out.leaves.ptr, bad);
     SOLHIP_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -1104,6 +1122,8 @@ int build_tlas(hipStream_t s, const solhip_instance *inst_dev, const BlasRef *re
         out.depth = 0;
         return 0;
     }
+    rc = reset_build_counters(s, sc);
+    if (rc) return rc;
     SOLHIP_HIP_CHECK(hipMemcpyAsync(sc.bounds.ptr, kOrdInit, 24, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_inst_boxes, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, inst_dev, refs_dev, n,
                        (float4 *)sc.boxes.ptr, (uint32_t *)sc.bounds.ptr);

@@ -11,6 +11,46 @@ namespace solhip {
 
 SOLHIP_HD uint32_t clz32(uint32_t x) { return x ? (uint32_t)__builtin_clz(x) : 32u; }
 
+// one geometry of a BLAS build, as the build kernels read it: vertex k of
+// BLAS-wide triangle t (tri_begin <= t < tri_begin + tri_count) is at
+// vbase + (first_vertex + i_k) * stride with i_k = idx[3 (t - tri_begin) + k],
+// or i_k = 3 (t - tri_begin) + k when idx is null (vbase then includes the
+// primitive offset); indices >= vertex_count are counted as errors
+struct GeomDesc {
+    const uint8_t *vbase;
+    const uint32_t *idx;
+    uint64_t stride;
+    uint32_t first_vertex, vertex_count;
+    uint32_t tri_begin, tri_count;
+};
+static_assert(sizeof(GeomDesc) == 40, "GeomDesc layout");
+
+// geometry of BLAS-wide triangle t: the last descriptor whose tri_begin <= t
+// (descriptors are in tri_begin order; one geometry needs no search)
+SOLHIP_HD uint32_t geom_index(const GeomDesc *geo, uint32_t n_geo, uint32_t t)
+{
+    uint32_t lo = 0, hi = n_geo;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (geo[mid].tri_begin <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// vertex of corner k of BLAS-wide triangle t in geometry G, relative to
+// G.first_vertex.  False when the index is >= vertex_count: iv is then the
+// last vertex (0 without vertices), so that malformed input cannot make the
+// build read outside the geometry's vertices, and the build fails.
+SOLHIP_HD bool tri_vertex_index(const GeomDesc &G, uint32_t t, int k, uint32_t &iv)
+{
+    const uint32_t local = t - G.tri_begin;
+    iv = G.idx ? G.idx[3ull * local + k] : 3u * local + (uint32_t)k;
+    if (iv < G.vertex_count) return true;
+    iv = G.vertex_count ? G.vertex_count - 1 : 0;
+    return false;
+}
+
 SOLHIP_HD uint32_t expand10(uint32_t v)
 {
     v &= 0x3FFu;

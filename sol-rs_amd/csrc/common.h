@@ -21,7 +21,10 @@
 //   when d[a] < 0) visits slots in the order s = i ^ oct, i = 0..7, which is
 //   near-to-far without sorting.
 //
-//   BLAS leaf record: TriRec (48 B) = v0.xyz v1.xyz v2.xyz, prim id, 2 pad words.
+//   BLAS leaf record: TriRec (48 B) = v0.xyz v1.xyz v2.xyz, BLAS-wide triangle
+//                     index (geometries numbered one after another; the
+//                     tie rule and the hit shaders use it), geometry index,
+//                     triangle index within the geometry (40 B read per test).
 //   TLAS leaf record: InstRec (64 B) = world-to-object 3x4 (48 B),
 //                     BLAS node pointer, BLAS leaf pointer; plus a parallel
 //                     uint32 inst_meta = instance index | mask << 24.
@@ -53,8 +56,9 @@ static_assert(sizeof(WideNode) == kWideNodeBytes, "WideNode must be 80 bytes");
 
 struct alignas(16) TriRec {
     float v[9];
-    uint32_t prim;
-    uint32_t pad[2];
+    uint32_t prim;    // BLAS-wide triangle index
+    uint32_t geom;    // geometry index (gl_GeometryIndexEXT)
+    uint32_t lprim;   // triangle index within the geometry (gl_PrimitiveID)
 };
 static_assert(sizeof(TriRec) == 48, "TriRec must be 48 bytes");
 

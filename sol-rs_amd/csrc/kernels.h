@@ -17,6 +17,13 @@ struct DevTables {
     uint32_t index_size;
 };
 
+// closest hit of a wavefront queue slot (20 B): the public solhip_hit
+// without the geometry index, which the programs' hit shaders do not read
+struct QHit {
+    float t, u, v;
+    uint32_t inst, prim;   // prim: BLAS-wide triangle index
+};
+
 struct Cam {
     float view_inverse[16];
     float projection_inverse[16];
@@ -137,7 +144,7 @@ struct WfArgs {
     float4 *st0, *st1, *psum;
     struct FixSum *fsum;
     QRay *q[2];                       // extension queues (ping-pong by bounce)
-    solhip_hit *hits;                 // closest hits, by slot of the current queue
+    QHit *hits;                       // closest hits, by slot of the current queue
     QRay *sq;                         // shadow queue
     float4 *scontrib;                 // shadow-ray contributions, by shadow slot
     uint32_t *counts;                 // [0], [1]: extension queue sizes, [2]: shadow queue size
@@ -154,6 +161,10 @@ struct WfArgs {
 
 int launch_trace_rays(hipStream_t s, const SceneView &sv, const solhip_ray *rays, solhip_hit *hits, uint32_t n,
                       uint32_t flags, Counters *counters, Timer *timer);
+// rewrites the BLAS-wide triangle index of every hit (geom = 0) into the
+// triangle index within its geometry and the geometry index, for instances
+// whose BLAS has several geometries (refs: the TLAS's BlasRef per instance)
+int launch_split_geometry(hipStream_t s, solhip_hit *hits, uint64_t n, const struct BlasRef *refs, uint32_t n_inst);
 int launch_ao(hipStream_t s, const ProgArgs &a, Timer *timer);
 // bytes of the persistent AO kernel's fetch heads
 uint64_t ao_head_bytes();

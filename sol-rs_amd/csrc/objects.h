@@ -94,10 +94,22 @@ struct Buffer : Obj {
     uint64_t bytes = 0;
 };
 
+// generation numbers: a BLAS or TLAS takes a new one at every (re)build, so
+// that what was built over it can tell that it changed
+uint64_t next_generation();
+
 struct Blas : Obj {
     DeviceBvh bvh;
     DevScratch scratch;
     uint32_t build_flags = 0;   // SOLHIP_BUILD_*, kept for rebuilds
+    uint32_t n_geo = 0;         // geometries of the last build
+    DevBuf geo_dev;             // GeomDesc[n_geo], then u32 [n_geo, tri_begin[n_geo]]
+    uint64_t gen = 0;
+    // the u32 table BlasRef::geo points to (nullptr with one geometry)
+    const uint32_t *geo_table() const
+    {
+        return n_geo > 1 ? (const uint32_t *)((const char *)geo_dev.ptr + (uint64_t)n_geo * sizeof(GeomDesc)) : nullptr;
+    }
 };
 
 struct Tlas : Obj {
@@ -106,7 +118,11 @@ struct Tlas : Obj {
     DevBuf inst_dev;   // solhip_instance[]
     DevBuf refs_dev;   // BlasRef[]
     std::vector<uint64_t> blas_handles;   // distinct BLAS referenced by the instances
+    std::vector<uint64_t> blas_gens;      // their generations when this TLAS was built
     uint32_t n_inst = 0;
+    bool multi_geo = false;               // some instance's BLAS has several geometries
+    bool built = false;                   // the last (re)build succeeded
+    uint64_t gen = 0;
     DevBuf inst_tris_dev;                 // u32 triangle count of each instance's BLAS
 };
 
@@ -181,7 +197,26 @@ int bad_arg(const std::string &what);
 Context *ctx_of(solhip_ctx c);
 hipStream_t stream_of(Context *cx, solhip_stream s, bool &ok);
 uint32_t stack_depth_of(const Tlas *t);
+// SOLHIP_OK, or SOLHIP_ERROR_STALE (with the message set) when a BLAS the
+// TLAS was built over has been rebuilt or destroyed since
+int check_fresh(const Tlas *t);
 SceneView view_of(const Tlas *t);
+// device scratch that solhip_trace_noalloc hands to the program instead of
+// the program's own buffers
+struct FixedScratch {
+    void *ptr;
+    uint64_t bytes;
+};
+// byte offsets of the AO wavefront's scratch within one allocation
+struct AoScratchLayout {
+    uint64_t qcount, heads, queue, occl, total;
+};
+AoScratchLayout ao_scratch_layout(uint64_t npix, uint32_t n_ao);
+// solhip_trace, with the AO program's scratch in `fixed` when it is given
+int trace_program(solhip_ctx ctx, solhip_stream st, solhip_program ph, solhip_tlas th,
+                  const solhip_scene_tables *tables, const solhip_scene_uniforms *u, int32_t accum_start_frame,
+                  const solhip_images *images, uint32_t W, uint32_t H, const FixedScratch *fixed);
+
 // NEE emitter table of a scene into p's buffers; *nl = number of lights
 int build_lights(hipStream_t s, const DevTables &tb, const Tlas *t, DevBuf &offsets, DevBuf &lights, uint32_t *nl,
                  Timer *timer);

@@ -47,7 +47,7 @@ struct ShadeOut {
 
 // Shades the hit h of ray r of path p.  s0 = (T, rng), s1 = (L, depth |
 // count_emission << 16) are updated in place; o_ gets the rays to queue.
-__device__ __forceinline__ void shade_vertex(const ShadeParams &P, uint32_t p, const QRay &r, const solhip_hit &h,
+__device__ __forceinline__ void shade_vertex(const ShadeParams &P, uint32_t p, const QRay &r, const QHit &h,
                                              float4 &s0, float4 &s1, ShadeOut &o_)
 {
     o_.sc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);

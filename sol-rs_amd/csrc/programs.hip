@@ -70,6 +70,7 @@ __device__ __forceinline__ bool ao_primary_result(const ProgArgs &A, uint32_t pi
         m.v = 0.0f;
         m.inst = SOLHIP_MISS;
         m.prim = SOLHIP_MISS;
+        m.geom = SOLHIP_MISS;
         solhip_hit *dst = A.hits_out + (uint64_t)A.W * A.H + (uint64_t)pix * A.n_ao;
         for (uint32_t k = 0; k < A.n_ao; ++k) dst[k] = m;
     }

@@ -73,7 +73,7 @@ struct SrvArgs {
     FixSum *fsum;         // per-sample streams: per-slot order-free sums (lane 0's count: lanes done)
     uint32_t *pmeta;      // job | sample << 8
     QRay *q[2];
-    solhip_hit *hits;
+    QHit *hits;
     QRay *sq;
     float4 *scontrib;
     uint32_t *retire;
@@ -215,7 +215,7 @@ __device__ __forceinline__ void srv_trace(const SrvArgs &A, const QRay *q, uint3
             const Hit h = trav_result(st);
             ++rays;
             if (!ANY) {
-                solhip_hit out;
+                QHit out;
                 out.t = h.t;
                 out.u = h.u;
                 out.v = h.v;
@@ -438,6 +438,7 @@ constexpr uint32_t kLaneShare = 4;
 
 struct SrvScene {
     SceneView sv;
+    uint64_t tlas = 0, tlas_gen = 0;   // the TLAS and its generation at registration
     DevTables tb;
     DevBuf light_offsets, lights;
     uint32_t nlights = 0;
@@ -480,7 +481,7 @@ struct Server : Obj {
         a.pmeta = (uint32_t *)pmeta.ptr;
         a.q[0] = (QRay *)q0.ptr;
         a.q[1] = (QRay *)q1.ptr;
-        a.hits = (solhip_hit *)hits.ptr;
+        a.hits = (QHit *)hits.ptr;
         a.sq = (QRay *)sq.ptr;
         a.scontrib = (float4 *)scontrib.ptr;
         a.retire = (uint32_t *)retire.ptr;
@@ -691,7 +692,7 @@ int32_t solhip_server_create(solhip_ctx ctx, uint64_t hbm_budget, uint32_t max_j
                 sv->st1.reserve(n * 16) || sv->psum.reserve(n * 16) || sv->fsum.reserve(n * sizeof(FixSum)) ||
                 sv->pmeta.reserve(n * 4) ||
                 sv->q0.reserve(n * sizeof(QRay)) || sv->q1.reserve(n * sizeof(QRay)) ||
-                sv->hits.reserve(n * sizeof(solhip_hit)) || sv->sq.reserve(n * sizeof(QRay)) ||
+                sv->hits.reserve(n * sizeof(QHit)) || sv->sq.reserve(n * sizeof(QRay)) ||
                 sv->scontrib.reserve(n * 16) || sv->retire.reserve(n * 4) || sv->counts.reserve(256) ||
                 sv->heads.reserve(2ull * kSrvParts * kHeadStride * 4) || sv->done.reserve((uint64_t)max_jobs * 4);
     if (!fail) fail = hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -726,6 +727,19 @@ int32_t solhip_server_destroy(solhip_ctx ctx, solhip_server h)
     return SOLHIP_OK;
 }
 
+// the scene's TLAS still exists with the generation it had at registration,
+// and its BLASes are those it was built over
+static int scene_fresh(const SrvScene *sc)
+{
+    const Tlas *t = get<Tlas>(sc->tlas, Kind::Tlas);
+    if (!t || t->gen != sc->tlas_gen) {
+        set_error(std::string("the scene's TLAS was ") + (t ? "rebuilt" : "destroyed") +
+                  " after solhip_server_scene registered it; register the scene again");
+        return SOLHIP_ERROR_STALE;
+    }
+    return check_fresh(t);
+}
+
 int32_t solhip_server_scene(solhip_ctx ctx, solhip_server h, solhip_tlas th, const solhip_scene_tables *tables,
                             uint32_t *scene_id)
 {
@@ -738,12 +752,15 @@ int32_t solhip_server_scene(solhip_ctx ctx, solhip_server h, solhip_tlas th, con
     if (!tables || !tables->instances_addr || !tables->vertex_table_addr || !tables->index_table_addr ||
         !tables->material_table_addr || (tables->index_size != 4 && tables->index_size != 8) || !scene_id)
         return bad_arg("scene tables");
+    if (const int rc = check_fresh(t)) return rc;
     if (stack_depth_of(t) > (uint32_t)kMaxStack) {
         set_error("acceleration structures too deep for the traversal stack");
         return SOLHIP_ERROR_UNSUPPORTED;
     }
     SrvScene *sc = new SrvScene();
     sc->sv = view_of(t);
+    sc->tlas = th;
+    sc->tlas_gen = t->gen;
     sc->tb.instances = (const solhip_scene_instance *)tables->instances_addr;
     sc->tb.vertex_table = (const uint64_t *)tables->vertex_table_addr;
     sc->tb.index_table = (const uint64_t *)tables->index_table_addr;
@@ -780,8 +797,11 @@ int32_t solhip_server_submit(solhip_ctx ctx, solhip_server h, const solhip_job_d
     if (!sv) return bad_handle("server");
     if (!job || !job_id) return bad_arg("job");
     if (job->scene >= sv->scenes.size()) return bad_arg("scene id");
+    if (const int rc = scene_fresh(sv->scenes[job->scene])) return rc;
     if (job->W == 0 || job->H == 0) return bad_arg("image size");
     const uint64_t npix = (uint64_t)job->W * job->H;
+    if (job->spec.sample_streams && (uint64_t)job->spec.spp_per_frame * npix > (1ull << 32))
+        return bad_arg("sample_streams: spp_per_frame * W * H must not exceed 2^32");
     if (npix > sv->max_slots) return bad_arg("image larger than the server's slot budget");
     if (job->spec.spp_per_frame == 0 || job->spec.spp_per_frame > 0xFFFFFFu || job->spec.max_depth > 4096 ||
         job->spec.enable_sky > 2 || job->spec.sample_streams > 1)
@@ -817,6 +837,13 @@ int32_t solhip_server_run(solhip_ctx ctx, solhip_server h, uint32_t max_steps, u
     if (!c) return bad_handle("context");
     Server *sv = get<Server>(h, Kind::Server);
     if (!sv) return bad_handle("server");
+    // a scene rebuilt since its registration would leave jobs reading freed
+    // or changed acceleration structures: refuse to run them
+    for (const auto &kv : sv->all) {
+        const SrvJob &jb = kv.second;
+        if (jb.state == SOLHIP_JOB_DONE) continue;
+        if (const int rc = scene_fresh(sv->scenes[jb.desc.scene])) return rc;
+    }
     for (uint32_t k = 0; k < max_steps; ++k) {
         int rc = sv->admit();
         if (rc) return rc;

@@ -77,6 +77,8 @@ __device__ __forceinline__ Surf fetch_surface(const DevTables &tb, uint32_t inst
     return s;
 }
 
+// a traversal result as a public hit record: geometry 0 (launch_split_geometry
+// splits the BLAS-wide triangle index of several-geometry BLASes afterwards)
 __device__ __forceinline__ void store_hit(solhip_hit *dst, const Hit &h)
 {
     solhip_hit o;
@@ -85,6 +87,7 @@ __device__ __forceinline__ void store_hit(solhip_hit *dst, const Hit &h)
     o.v = h.v;
     o.inst = h.inst;
     o.prim = h.prim;
+    o.geom = h.inst == SOLHIP_MISS ? SOLHIP_MISS : 0u;
     *dst = o;
 }
 

@@ -1,6 +1,7 @@
 // trace.hip -- raw ray queries (solhip_trace_rays) and small utility kernels.
 #include <hip/hip_runtime.h>
 
+#include "bvh.h"
 #include "kernels.h"
 #include "traverse.h"
 #include "traverse_step.h"
@@ -61,6 +62,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_rays(SceneView sv, const 
         o.v = h.v;
         o.inst = h.inst;
         o.prim = h.prim;
+        o.geom = h.inst == SOLHIP_MISS ? SOLHIP_MISS : 0u;
         hits[i] = o;
     }
     const uint64_t ov = __ballot(active && tc.overflow);
@@ -94,6 +96,39 @@ int launch_trace_rays(hipStream_t s, const SceneView &sv, const solhip_ray *rays
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         set_error(std::string("trace launch: ") + hipGetErrorString(e));
+        return SOLHIP_ERROR_DEVICE;
+    }
+    return 0;
+}
+
+// (geometry, triangle in geometry) of hits on several-geometry BLASes: the
+// geometry is the last one whose first BLAS-wide triangle is <= prim
+__global__ void k_split_geometry(solhip_hit *hits, uint64_t n, const BlasRef *refs, uint32_t n_inst)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    solhip_hit h = hits[i];
+    if (h.inst >= n_inst) return;   // a miss
+    const uint32_t *geo = refs[h.inst].geo;
+    if (!geo) return;
+    uint32_t lo = 0, hi = geo[0];
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (geo[1 + mid] <= h.prim) lo = mid;
+        else hi = mid;
+    }
+    h.prim -= geo[1 + lo];
+    h.geom = lo;
+    hits[i] = h;
+}
+
+int launch_split_geometry(hipStream_t s, solhip_hit *hits, uint64_t n, const BlasRef *refs, uint32_t n_inst)
+{
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_split_geometry, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, hits, n, refs, n_inst);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(std::string("split-geometry launch: ") + hipGetErrorString(e));
         return SOLHIP_ERROR_DEVICE;
     }
     return 0;

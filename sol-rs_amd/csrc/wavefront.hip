@@ -177,7 +177,7 @@ template <bool ANY>
 __device__ __forceinline__ void queue_result(const WfArgs &A, uint32_t i, uint32_t path, const Hit &h)
 {
     if (!ANY) {
-        solhip_hit out;
+        QHit out;
         out.t = h.t;
         out.u = h.u;
         out.v = h.v;
@@ -460,7 +460,7 @@ __global__ void k_pt_shade_nrc(WfArgs A, uint32_t b)
             const QRay r = qin[i];
             if (r.path != kNoPath) {
                 const uint32_t p = r.path;
-                const solhip_hit h = A.hits[i];
+                const QHit h = A.hits[i];
                 float4 s0 = A.st0[p], s1 = A.st1[p];
                 const uint32_t depth = as_u(s1.w) & 0xFFFFu;
                 bool shade = true;

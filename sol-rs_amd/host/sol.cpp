@@ -251,16 +251,22 @@ BLAS::BLAS(std::shared_ptr<Context> ctx, CommandBuffer &cmd, const std::vector<G
         s.vertex_addr = gi.vertex_buffer;
         s.vertex_stride = vertex_stride;
         s.opaque = 1;
+        s.first_vertex = gi.vertex_offset;
+        s.vertex_count = gi.vertex_count;
         if (gi.index_buffer) {
             // indexed: BuildRangeInfo{primitive_count = index_count / 3, primitive_offset =
             // index byte offset, first_vertex = vertex_offset} (src/ray/acceleration.rs:182-200)
-            s.first_vertex = gi.vertex_offset;
-            s.vertex_count = gi.vertex_count;
             s.index_addr = *gi.index_buffer;
-            s.index_byte_offset = *gi.index_offset_size;
+            s.primitive_offset = *gi.index_offset_size;
             s.index_count = *gi.index_count;
         } else {
-            throw Error(SOLHIP_ERROR_UNSUPPORTED, "BLAS::new: non-indexed geometry is not supported");
+            // non-indexed: primitive_count = vertex_count / 3, primitive_offset = the
+            // vertex byte offset, first_vertex = vertex_offset (acceleration.rs:185-188,
+            // 194-199).  Vulkan applies both, so a primitive whose vertices start at
+            // vertex_offset > 0 is read from 2 * vertex_offset on; kept as the reference has it.
+            s.index_addr = 0;
+            s.primitive_offset = gi.vertex_offset_size;
+            s.index_count = 0;
         }
         g.push_back(s);
     }

@@ -20,7 +20,7 @@ TRACE_COUNT = 0x100
 BUILD_PREFER_FAST_TRACE = 0x1
 
 ERRORS = {0: "OK", -1: "INVALID_ARGUMENT", -2: "INVALID_HANDLE", -3: "OUT_OF_MEMORY", -4: "NO_DEVICE",
-          -5: "DEVICE", -6: "UNSUPPORTED", -7: "OVERFLOW"}
+          -5: "DEVICE", -6: "UNSUPPORTED", -7: "OVERFLOW", -8: "STALE"}
 
 
 class SolhipError(RuntimeError):
@@ -31,7 +31,7 @@ class SolhipError(RuntimeError):
 
 class Geometry(C.Structure):
     _fields_ = [("vertex_addr", C.c_uint64), ("vertex_stride", C.c_uint64), ("first_vertex", C.c_uint32),
-                ("vertex_count", C.c_uint32), ("index_addr", C.c_uint64), ("index_byte_offset", C.c_uint64),
+                ("vertex_count", C.c_uint32), ("index_addr", C.c_uint64), ("primitive_offset", C.c_uint64),
                 ("index_count", C.c_uint32), ("opaque", C.c_uint32)]
 
 
@@ -243,17 +243,38 @@ class Context:
         check(lib().solhip_stream_create(self.h, C.byref(h)))
         return h.value
 
-    def blas_build(self, vbuf, stride, vertex_count, ibuf, index_count, first_vertex=0, index_byte_offset=0,
-                   stream=0, flags=0):
-        """flags: BUILD_PREFER_FAST_TRACE (binned SAH) or 0 (Morton order)"""
-        g = Geometry(vbuf.addr, stride, first_vertex, vertex_count, ibuf.addr, index_byte_offset, index_count, 1)
+    @staticmethod
+    def geometry(vbuf, stride, vertex_count, ibuf=None, index_count=0, first_vertex=0, primitive_offset=0):
+        """one solhip_geometry: indexed when ibuf is given, else non-indexed
+        (vertex_count / 3 triangles)"""
+        return Geometry(vbuf.addr, stride, first_vertex, vertex_count, ibuf.addr if ibuf is not None else 0,
+                        primitive_offset, index_count if ibuf is not None else 0, 1)
+
+    def blas_build_geometries(self, geoms, stream=0, flags=0):
+        """BLAS over several geometries (BLAS::new's Vec<GeometryInstance>)"""
+        arr = (Geometry * len(geoms))(*geoms)
         h = C.c_uint64()
-        check(lib().solhip_blas_build(self.h, stream, C.byref(g), 1, flags, C.byref(h)))
+        check(lib().solhip_blas_build(self.h, stream, arr, len(geoms), flags, C.byref(h)))
         return h.value
 
+    def blas_rebuild_geometries(self, blas, geoms, stream=0):
+        arr = (Geometry * len(geoms))(*geoms)
+        check(lib().solhip_blas_rebuild(self.h, stream, blas, arr, len(geoms)))
+
+    def blas_build(self, vbuf, stride, vertex_count, ibuf, index_count, first_vertex=0, primitive_offset=0,
+                   stream=0, flags=0):
+        """flags: BUILD_PREFER_FAST_TRACE (binned SAH) or 0 (Morton order)"""
+        g = self.geometry(vbuf, stride, vertex_count, ibuf, index_count, first_vertex, primitive_offset)
+        return self.blas_build_geometries([g], stream, flags)
+
     def blas_rebuild(self, blas, vbuf, stride, vertex_count, ibuf, index_count, stream=0):
-        g = Geometry(vbuf.addr, stride, 0, vertex_count, ibuf.addr, 0, index_count, 1)
-        check(lib().solhip_blas_rebuild(self.h, stream, blas, C.byref(g), 1))
+        self.blas_rebuild_geometries(blas, [self.geometry(vbuf, stride, vertex_count, ibuf, index_count)], stream)
+
+    def blas_destroy(self, blas):
+        check(lib().solhip_blas_destroy(self.h, blas))
+
+    def tlas_destroy(self, tlas):
+        check(lib().solhip_tlas_destroy(self.h, tlas))
 
     def tlas_build(self, instances, stream=0):
         inst = np.ascontiguousarray(instances, INSTANCE_DESC)
@@ -340,11 +361,22 @@ class GpuScene:
         self.sd = sd
         self.bufs = []
         self.blas = []
+        self.blas_geoms = []   # per BLAS: its solhip_geometry records (kept for rebuilds)
         for b in sd.blas:
-            vb = ctx.upload(b.vertices)
-            ib = ctx.upload(b.indices.astype(np.uint32))
-            self.bufs += [vb, ib]
-            self.blas.append(ctx.blas_build(vb, 64, len(b.vertices), ib, len(b.indices), flags=build_flags))
+            # a scenes.Blas is one indexed geometry; a scenes.MultiBlas lists
+            # (vertices, indices or None) per geometry
+            geoms = []
+            for verts, inds in (getattr(b, "geometries", None) or [(b.vertices, b.indices)]):
+                vb = ctx.upload(verts)
+                self.bufs.append(vb)
+                if inds is None:
+                    geoms.append(ctx.geometry(vb, 64, len(verts)))
+                else:
+                    ib = ctx.upload(np.asarray(inds).astype(np.uint32))
+                    self.bufs.append(ib)
+                    geoms.append(ctx.geometry(vb, 64, len(verts), ib, len(inds)))
+            self.blas_geoms.append(geoms)
+            self.blas.append(ctx.blas_build_geometries(geoms, flags=build_flags))
         descs = sd.instance_descs.copy()
         descs["blas"] = [self.blas[int(k)] for k in sd.instance_descs["blas"]] if len(descs) else []
         self.descs = descs
@@ -368,6 +400,18 @@ class GpuScene:
         self.bufs += [self.inst_buf, self.vt, self.it, self.mat_buf, self.mt]
         self.tables = SceneTables(self.inst_buf.addr, len(sd.scene_instances), index_size, self.vt.addr, self.it.addr,
                                   self.mt.addr, len(sd.objects), 0)
+
+    def rebuild_blas(self, k):
+        """BLAS k again from its resident geometry (its TLAS then needs a rebuild)"""
+        self.ctx.blas_rebuild_geometries(self.blas[k], self.blas_geoms[k])
+
+    def move_instances(self, sd_moved):
+        """TLAS::regenerate after SceneDescription::blas_transforms + update
+        (src/ray/mod.rs:162-193): the TLAS rebuilt from the moved scene's
+        instance records and its SceneInstance table uploaded in place"""
+        self.ctx.sync()
+        self.inst_buf.upload(np.ascontiguousarray(sd_moved.scene_instances, SCENE_INSTANCE))
+        self.update_instances(sd_moved.instance_descs)
 
     def update_instances(self, instance_descs):
         descs = instance_descs.copy()

@@ -30,7 +30,8 @@ UNIFORMS = np.dtype([("model", "<f4", 16), ("view", "<f4", 16), ("view_inverse",
 
 # raw ray / hit records of the trace_rays entry point
 RAY = np.dtype([("o", "<f4", 3), ("tmin", "<f4"), ("d", "<f4", 3), ("tmax", "<f4")])
-HIT = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("inst", "<u4"), ("prim", "<u4")])
+# hit: geom = gl_GeometryIndexEXT, prim = gl_PrimitiveID (within the geometry)
+HIT = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("inst", "<u4"), ("prim", "<u4"), ("geom", "<u4")])
 MISS = 0xFFFFFFFF
 
 # VkGeometryInstanceFlagBitsKHR used by TLAS::create_instances
@@ -44,4 +45,4 @@ assert SCENE_INSTANCE.itemsize == 144
 assert INSTANCE_DESC.itemsize == 64
 assert UNIFORMS.itemsize == 400
 assert RAY.itemsize == 32
-assert HIT.itemsize == 20
+assert HIT.itemsize == 24

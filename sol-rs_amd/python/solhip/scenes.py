@@ -177,6 +177,33 @@ def assemble(name, blas_list, inst_blas, inst_xforms, inst_material, materials, 
     return SceneData(name, blas_list, descs, sis, objects, mats, inst_material, camera, extent)
 
 
+def with_transforms(sd, inst_xforms):
+    """the scene with its instances moved to new 4x4 object-to-world
+    matrices: the TLAS records and the SceneInstance table (transform and
+    transform_it, what SceneDescription::blas_transform / update write,
+    src/ray/mod.rs:162-193) are recomputed as assemble() makes them"""
+    descs = sd.instance_descs.copy()
+    sis = sd.scene_instances.copy()
+    for i, m in enumerate(inst_xforms):
+        m = np.asarray(m, np.float64)
+        descs[i]["transform"] = affine_rows(m)
+        m32 = m.astype(np.float32).astype(np.float64)
+        sis[i]["transform"] = colmajor(m32)
+        sis[i]["transform_it"] = colmajor(np.linalg.inv(m32).T)
+    return SceneData(sd.name, sd.blas, descs, sis, sd.objects, sd.materials, sd.instance_material, sd.camera,
+                     sd.extent)
+
+
+def instance_xforms(sd):
+    """the instances' 4x4 object-to-world matrices (float64)"""
+    out = []
+    for d in sd.instance_descs:
+        m = np.eye(4)
+        m[:3, :4] = d["transform"].reshape(3, 4)
+        out.append(m)
+    return out
+
+
 def quad_tris(p0, p1, p2, p3):
     """two triangles (p0,p1,p2), (p0,p2,p3)"""
     return [p0, p1, p2, p0, p2, p3]
@@ -448,3 +475,73 @@ def blue_noise_synthetic(size=256, seed=7):
     asset is not available (white noise, same shape and format)."""
     rng = np.random.default_rng(seed)
     return rng.integers(0, 256, size=(size, size, 4), dtype=np.uint8)
+
+
+class MultiBlas:
+    """A BLAS over several geometries (BLAS::new's Vec<GeometryInstance>,
+    src/ray/acceleration.rs:136-208): each geometry is (ModelVertex array,
+    u32 indices) or (ModelVertex array, None) for non-indexed triangles
+    (vertices 3i..3i+2, vertex_count / 3 triangles).  Its triangles are
+    numbered geometry after geometry; `vertices` / `indices` give the hit
+    shaders' object view of that numbering (the geometries' vertices back to
+    back, indices rebased onto them)."""
+
+    def __init__(self, geometries):
+        self.geometries = [(np.ascontiguousarray(v, MODEL_VERTEX),
+                            None if i is None else np.ascontiguousarray(i, np.uint32).reshape(-1))
+                           for v, i in geometries]
+        verts, inds, base = [], [], 0
+        for v, i in self.geometries:
+            n = (len(v) // 3) * 3
+            local = np.arange(n, dtype=np.uint32) if i is None else i
+            verts.append(v)
+            inds.append(local.astype(np.uint64) + base)
+            base += len(v)
+        self.vertices = np.concatenate(verts) if verts else np.zeros(0, MODEL_VERTEX)
+        self.indices = (np.concatenate(inds) if inds else np.zeros(0, np.uint64)).astype(np.uint32)
+
+    @property
+    def tri_count(self):
+        return len(self.indices) // 3
+
+    @property
+    def geo_begin(self):
+        """first BLAS-wide triangle of each geometry"""
+        out, t = [], 0
+        for v, i in self.geometries:
+            out.append(t)
+            t += (len(v) // 3) if i is None else len(i) // 3
+        return out
+
+
+def multi_geometry_scene(seed=9):
+    """two instances of a 3-geometry BLAS (two indexed soups, one
+    non-indexed soup) plus an instance of a one-geometry BLAS, for the
+    geometry index of hits (gl_GeometryIndexEXT)"""
+    rng = np.random.default_rng(seed)
+
+    def soup(n, c):
+        p = c + rng.uniform(-0.6, 0.6, size=(n, 1, 3)) + rng.uniform(-0.25, 0.25, size=(n, 3, 3))
+        return p.reshape(-1, 3)
+    g0 = soup(40, np.array([-0.8, 0.0, 0.0]))
+    g1 = soup(25, np.array([0.8, 0.2, 0.0]))
+    g2 = soup(30, np.array([0.0, -0.6, 0.3]))
+    # indexed geometries with shuffled, shared vertices; the non-indexed one in triangle order
+    geoms = []
+    for p in (g0, g1):
+        uniq, inv = np.unique(np.round(p, 6), axis=0, return_inverse=True)
+        perm = rng.permutation(len(uniq))
+        v = make_vertices(uniq[perm])
+        rank = np.empty(len(perm), np.int64)
+        rank[perm] = np.arange(len(perm))
+        geoms.append((v, rank[inv.reshape(-1)].astype(np.uint32)))
+    geoms.append((make_vertices(g2, flat_normals(g2)), None))
+    mb = MultiBlas(geoms)
+    single = soup_blas(soup(20, np.array([0.0, 0.9, -0.4])))
+    xf = [np.eye(4), np.eye(4), np.eye(4)]
+    xf[1][:3, 3] = (0.3, -0.2, -1.5)
+    xf[1][:3, :3] = random_rotation(rng)
+    xf[2][:3, 3] = (0.0, 0.0, 0.2)
+    cam = Camera((0.0, 0.0, 4.0), (0.0, 0.0, 0.0))
+    return assemble("multi_geometry", [mb, single], [0, 0, 1], xf, [0, 1, 2],
+                    [material((0.8, 0.7, 0.6)), material((0.5, 0.6, 0.7)), material((0.3, 0.8, 0.4))], cam, 3.0)

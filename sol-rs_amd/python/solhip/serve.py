@@ -63,6 +63,8 @@ def run_load(ctx, gpu_scenes, scene_data, reqs, hbm_budget=8 << 30, max_jobs=64,
     lat = np.array([srv.job(j).done_ms - srv.job(j).submit_ms for j, _ in jobs])
     wait = np.array([srv.job(j).admit_ms - srv.job(j).submit_ms for j, _ in jobs])
     srv.close()
+    for b in bufs.values():
+        b.destroy()
     return {"jobs": len(jobs), "makespan_s": makespan, "jobs_per_s": len(jobs) / makespan,
             "mrays_per_s": rays / makespan / 1e6, "rays": int(rays),
             "latency_ms_p50": float(np.percentile(lat, 50)), "latency_ms_p99": float(np.percentile(lat, 99)),

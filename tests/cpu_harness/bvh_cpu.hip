@@ -17,6 +17,14 @@ using namespace solhip;
 
 namespace {
 
+// a traversal result as the 24-B public hit record (one geometry per BLAS)
+static void put_hit(void *hits, size_t i, const Hit &h)
+{
+    const uint32_t geom = h.inst == SOLHIP_MISS ? SOLHIP_MISS : 0u;
+    std::memcpy((char *)hits + 24 * i, &h, 20);
+    std::memcpy((char *)hits + 24 * i + 20, &geom, 4);
+}
+
 struct HostBvh {
     std::vector<WideNode> nodes;
     std::vector<uint32_t> leaf_ids;
@@ -268,7 +276,8 @@ void *hc_scene_create(const float *const *tris, const uint32_t *ntris, uint32_t 
             TriRec r;
             std::memcpy(r.v, tris[b] + 9 * (size_t)p, 36);
             r.prim = p;
-            r.pad[0] = r.pad[1] = 0;
+            r.geom = 0;
+            r.lprim = p;
             s->blas[b].tris[i] = r;
         }
     }
@@ -331,7 +340,7 @@ void hc_trace(void *h, const float *rays, uint32_t n, uint32_t any, void *hits, 
         TravCounts tc{0, 0, 0, false};
         Hit hit = any ? trace_ray<true, true>(sv, v3(r[0], r[1], r[2]), v3(r[4], r[5], r[6]), r[3], r[7], 0xFFu, lds, 1, tc)
                       : trace_ray<false, true>(sv, v3(r[0], r[1], r[2]), v3(r[4], r[5], r[6]), r[3], r[7], 0xFFu, lds, 1, tc);
-        std::memcpy((char *)hits + 20 * (size_t)i, &hit, 20);
+        put_hit(hits, i, hit);
         counts[0] += tc.nodes;
         counts[1] += tc.tris;
         counts[2] += tc.insts;
@@ -375,7 +384,7 @@ void hc_trace_steps(void *h, const float *rays, uint32_t n, uint32_t any, void *
                                   : trav_step<false, true>(st[l], sv, 0xFFu, lds.data() + l, kLanes, tc[l]);
             if (done) {
                 const Hit hit = trav_result(st[l]);
-                std::memcpy((char *)hits + 20 * (size_t)rid[l], &hit, 20);
+                put_hit(hits, rid[l], hit);
                 counts[0] += tc[l].nodes;
                 counts[1] += tc[l].tris;
                 counts[2] += tc[l].insts;
@@ -385,6 +394,30 @@ void hc_trace_steps(void *h, const float *rays, uint32_t n, uint32_t any, void *
         }
         if (!any_live && cursor >= n) break;
     }
+}
+
+// The BLAS build's vertex fetch replayed on the host over n_geo geometry
+// descriptors (host pointers): positions of every BLAS-wide triangle (9
+// floats each, as k_tri_boxes / k_write_tris read them) and the geometry and
+// in-geometry index of each; returns the number of out-of-range references
+// the build would count.
+uint32_t hc_geometry_fetch(const GeomDesc *geo, uint32_t n_geo, uint32_t n_tris, float *pos, uint32_t *geom,
+                           uint32_t *lprim)
+{
+    uint32_t bad = 0;
+    for (uint32_t t = 0; t < n_tris; ++t) {
+        const uint32_t g = geom_index(geo, n_geo, t);
+        const GeomDesc &G = geo[g];
+        for (int k = 0; k < 3; ++k) {
+            uint32_t iv;
+            if (!tri_vertex_index(G, t, k, iv)) ++bad;
+            const float *q = reinterpret_cast<const float *>(G.vbase + (uint64_t)(G.first_vertex + iv) * G.stride);
+            std::memcpy(pos + 9 * (size_t)t + 3 * k, q, 12);
+        }
+        geom[t] = g;
+        lprim[t] = t - G.tri_begin;
+    }
+    return bad;
 }
 
 }  // extern "C"

@@ -26,6 +26,8 @@ def lib():
         L.hc_trace.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, vp]
         L.hc_trace_steps.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, vp]
         L.hc_set_build_flags.argtypes = [C.c_uint32]
+        L.hc_geometry_fetch.restype = C.c_uint32
+        L.hc_geometry_fetch.argtypes = [vp, C.c_uint32, C.c_uint32, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -73,3 +75,42 @@ class HostScene:
         if getattr(self, "h", None):
             lib().hc_scene_destroy(self.h)
             self.h = None
+
+
+GEOM_DESC = np.dtype([("vbase", "<u8"), ("idx", "<u8"), ("stride", "<u8"), ("first_vertex", "<u4"),
+                      ("vertex_count", "<u4"), ("tri_begin", "<u4"), ("tri_count", "<u4")])
+
+
+def geometry_fetch(geoms):
+    """the BLAS build's vertex fetch over geometries given as dicts
+    (vertices: ModelVertex array, indices: u32 array or None, first_vertex,
+    vertex_count, primitive_offset in bytes): (positions (n, 3, 3), geometry
+    index and in-geometry index per triangle, out-of-range references)"""
+    keep = []
+    d = np.zeros(len(geoms), GEOM_DESC)
+    t = 0
+    for k, g in enumerate(geoms):
+        v = np.ascontiguousarray(g["vertices"])
+        keep.append(v)
+        vc = g.get("vertex_count", len(v))
+        d[k]["stride"] = v.dtype.itemsize
+        d[k]["first_vertex"] = g.get("first_vertex", 0)
+        d[k]["vertex_count"] = vc
+        d[k]["tri_begin"] = t
+        if g.get("indices") is not None:
+            i = np.ascontiguousarray(g["indices"], np.uint32)
+            keep.append(i)
+            d[k]["vbase"] = v.ctypes.data
+            d[k]["idx"] = i.ctypes.data
+            n = len(i) // 3
+        else:
+            d[k]["vbase"] = v.ctypes.data + g.get("primitive_offset", 0)
+            d[k]["idx"] = 0
+            n = vc // 3
+        d[k]["tri_count"] = n
+        t += n
+    pos = np.zeros((t, 3, 3), np.float32)
+    geom = np.zeros(t, np.uint32)
+    lprim = np.zeros(t, np.uint32)
+    bad = lib().hc_geometry_fetch(d.ctypes.data, len(geoms), t, pos.ctypes.data, geom.ctypes.data, lprim.ctypes.data)
+    return pos, geom, lprim, int(bad)

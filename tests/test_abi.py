@@ -42,10 +42,15 @@ def test_record_sizes_match_the_reference():
     assert C.sizeof(api.Geometry) == 48
     assert C.sizeof(api.SceneTables) == 48
     assert C.sizeof(api.Spec) == 72
+    # solhip_hit = {t, u, v, gl_InstanceID, gl_PrimitiveID, gl_GeometryIndexEXT}
+    assert records.HIT.itemsize == 24 and records.HIT.names[-1] == "geom"
+    # solhip_geometry: BuildRangeInfo::primitive_offset at byte 32, after index_addr
+    assert api.Geometry.primitive_offset.offset == 32 and api.Geometry.index_count.offset == 40
 
 
 def test_abi_version_and_spec_defaults():
-    assert api.lib().solhip_abi_version() == 1
+    assert api.lib().solhip_abi_version() == 2
+    assert api.ERRORS[-8] == "STALE"
     s = solhip.spec_default(solhip.PROG_PATHTRACE_GLSL)
     assert (s.spp_per_frame, s.max_depth, s.enable_sky) == (8, 32, 0)      # pathtrace.rgen:43-44, .rmiss:5
     s = solhip.spec_default(solhip.PROG_AO_GLSL)

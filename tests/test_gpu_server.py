@@ -1,10 +1,10 @@
 """The render-request server (solhip_server_*) on the device.
 
 Jobs of mixed scene, image size, sample count and estimator settings share
-one wavefront.  Each job's image must be bit-identical to a solhip_trace of
-SOLHIP_PROG_PATHTRACE with the same spec and uniforms (which is itself pinned
-to the oracle), whatever else runs beside it and whatever the admission
-order."""
+one wavefront.  Each job's image must be bit-identical to the oracle's
+rendering of the same estimator, spec and uniforms (sr_render_pathtrace), and
+to a solhip_trace of SOLHIP_PROG_PATHTRACE, whatever else runs beside it and
+whatever the admission order."""
 import numpy as np
 import pytest
 
@@ -63,11 +63,26 @@ def _run_jobs(ctx, budget, max_jobs=64):
     return srv, sds, gss, jobs
 
 
+def _oracle_render(sd, kw, W, H, frame):
+    """the job's image by the oracle: sr_render_pathtrace with the job's
+    estimator settings into a zeroed accumulator, accum_start 0"""
+    import solref
+    s = _spec(**kw)
+    cfg = solref.PTConfig(s.enable_sky, s.spp_per_frame, s.max_depth, s.rr_start, s.nee, s.sample_streams)
+    a = np.zeros((H, W, 4), np.float32)
+    o = np.zeros((H, W, 4), np.uint8)
+    solref.Scene(sd).render_pathtrace(sd.uniforms(W, H, frame), cfg, 0, a, o, W, H)
+    return a, o
+
+
 def _check(ctx, sds, gss, jobs):
     for jid, name, W, H, kw, frame, acc, out in jobs:
-        a_ref, o_ref = _eager(ctx, gss[name], sds[name], _spec(**kw), W, H, frame)
         a = acc.download(np.float32).reshape(H, W, 4)
         o = out.download(np.uint8).reshape(H, W, 4)
+        a_or, o_or = _oracle_render(sds[name], kw, W, H, frame)
+        assert (a.view(np.uint32) != a_or.view(np.uint32)).sum() == 0, ("oracle", jid, name, kw)
+        assert np.array_equal(o, o_or), ("oracle", jid, name, kw)
+        a_ref, o_ref = _eager(ctx, gss[name], sds[name], _spec(**kw), W, H, frame)
         assert (a.view(np.uint32) != a_ref.view(np.uint32)).sum() == 0, (jid, name, kw)
         assert np.array_equal(o, o_ref), (jid, name, kw)
 

@@ -23,7 +23,7 @@ def _check(ctx, oracle, sd, rays, brute=True, flags=0, build=0):
     if flags & solhip.TRACE_TERMINATE_ON_FIRST_HIT:
         assert np.array_equal(g["inst"] == 0xFFFFFFFF, r["inst"] == 0xFFFFFFFF)
         return g, r
-    diff = np.nonzero((g.view(np.uint32).reshape(-1, 5) != r.view(np.uint32).reshape(-1, 5)).any(axis=1))[0]
+    diff = np.nonzero((g.view(np.uint32).reshape(-1, 6) != r.view(np.uint32).reshape(-1, 6)).any(axis=1))[0]
     assert len(diff) == 0, "%d/%d hits differ, first %s: gpu %s ref %s" % (
         len(diff), len(rays), diff[:5], g[diff[:3]], r[diff[:3]])
     return g, r

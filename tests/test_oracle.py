@@ -610,3 +610,36 @@ def test_debug_program_barycentrics_and_misses(oracle):
     assert np.allclose(out[inside][:, :3].sum(-1), 1.0, atol=1e-6)
     assert np.allclose(out[outside][:, :3], d[outside], atol=1e-6)
     assert (out[..., 3] == 0.0).all()
+
+
+def test_oracle_hits_name_geometry_and_primitive(oracle):
+    """a BLAS of three geometries (two indexed, one non-indexed): the BVH
+    answer equals brute force, and every hit's (geometry, primitive within
+    the geometry) names a triangle that the ray hits at the reported t
+    (gl_GeometryIndexEXT / gl_PrimitiveID, src/ray/acceleration.rs:148-208)"""
+    sd = scenes.multi_geometry_scene()
+    sc = oracle.Scene(sd)
+    rays = np.concatenate([raygen.random_rays(3000, (0, 0, 0), 2.5, seed=21), raygen.aimed_rays(sd, 2000, seed=22)])
+    a = sc.trace(rays, use_bvh=False)
+    b = sc.trace(rays, use_bvh=True)
+    assert a.tobytes() == b.tobytes()
+    hit = a["inst"] != oracle.MISS
+    assert (a["geom"][~hit] == oracle.MISS).all()
+    mb = sd.blas[0]
+    geo_hits = a[hit & (sd.instance_descs["blas"][np.minimum(a["inst"], len(sd.instance_descs) - 1)] == 0)]
+    assert set(np.unique(geo_hits["geom"]).tolist()) == {0, 1, 2}
+    for h, r in zip(a[hit], rays[hit]):
+        d = sd.instance_descs[h["inst"]]
+        blas = sd.blas[int(d["blas"])]
+        verts, inds = (blas.geometries[h["geom"]] if hasattr(blas, "geometries") else (blas.vertices, blas.indices))
+        k = 3 * int(h["prim"])
+        tri = verts["pos"][:, :3][inds[k:k + 3]] if inds is not None else verts["pos"][k:k + 3, :3]
+        # the ray in object space with the oracle's fp32 inverse, evaluated
+        # left to right as the traversal does
+        m = oracle.instance_inverse(d["transform"]).astype(np.float32)
+        ro, rd = r["o"].astype(np.float32), r["d"].astype(np.float32)
+        o = np.array([((m[4 * a] * ro[0] + m[4 * a + 1] * ro[1]) + m[4 * a + 2] * ro[2]) + m[4 * a + 3]
+                      for a in range(3)], np.float32)
+        dd = np.array([(m[4 * a] * rd[0] + m[4 * a + 1] * rd[1]) + m[4 * a + 2] * rd[2] for a in range(3)], np.float32)
+        ok, t, u, v = oracle.intersect_triangle(o, dd, float(r["tmin"]), float(r["tmax"]), *tri)
+        assert ok and (np.float32(t), np.float32(u), np.float32(v)) == (h["t"], h["u"], h["v"])
