@@ -295,6 +295,24 @@ int32_t solhip_trace(solhip_ctx ctx, solhip_stream s, solhip_program p, solhip_t
                      const solhip_scene_tables *tables, const solhip_scene_uniforms *uniforms,
                      int32_t accum_start_frame, const solhip_images *images, uint32_t W, uint32_t H);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops that forbid
+ * allocation): solhip_trace with the same arguments and the same results, but
+ * the AO program's scratch (queue, fetch heads, per-ray occlusion) lives in
+ * one fixed device array of SOLHIP_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process ends, and no device memory is allocated.  Needs
+ * 256 + 8192 + W*H*(32 + n_ao) bytes (1920x1080 with up to 32 AO rays per
+ * pixel); larger frames fail with SOLHIP_ERROR_INVALID_ARGUMENT.  The
+ * megakernel programs (AO_GLSL, PATHTRACE_GLSL, DEBUG) use no scratch and
+ * run as in solhip_trace; the wavefront PATHTRACE program, whose path state
+ * is sized per image, is SOLHIP_ERROR_UNSUPPORTED.  The array keeps the last
+ * call's contents, and every call on one GPU shares it: calls that overlap in
+ * time on one GPU (two streams or host threads) race. */
+#define SOLHIP_NOALLOC_SCRATCH_BYTES (128ull << 20)
+int32_t solhip_trace_noalloc(solhip_ctx ctx, solhip_stream s, solhip_program p, solhip_tlas t,
+                             const solhip_scene_tables *tables, const solhip_scene_uniforms *uniforms,
+                             int32_t accum_start_frame, const solhip_images *images, uint32_t W, uint32_t H);
+
 /* counters accumulated since the last reset (rays, fetches, overflows) */
 int32_t solhip_stats(solhip_ctx ctx, solhip_counters *out, uint32_t reset);
 

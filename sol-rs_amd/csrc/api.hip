@@ -550,6 +550,7 @@ int32_t solhip_context_create(int32_t device, uint32_t flags, solhip_ctx *out)
         return SOLHIP_ERROR_NO_DEVICE;
     }
     SOLHIP_HIP_CHECK(hipSetDevice(device));
+    if (const int rc = noalloc_scratch_init(device)) return rc;
     Context *c = new Context();
     c->kind = Kind::Context;
     c->device = device;

@@ -200,6 +200,9 @@ uint32_t stack_depth_of(const Tlas *t);
 // SOLHIP_OK, or SOLHIP_ERROR_STALE (with the message set) when a BLAS the
 // TLAS was built over has been rebuilt or destroyed since
 int check_fresh(const Tlas *t);
+// device address of the fixed scratch of solhip_trace_noalloc on `device`
+// (the current device), taken once per device
+int noalloc_scratch_init(int device);
 SceneView view_of(const Tlas *t);
 // device scratch that solhip_trace_noalloc hands to the program instead of
 // the program's own buffers

@@ -18,6 +18,8 @@ PROG_AO_GLSL, PROG_PATHTRACE_GLSL, PROG_DEBUG, PROG_AO, PROG_PATHTRACE = 1, 2, 3
 TRACE_TERMINATE_ON_FIRST_HIT = 0x4
 TRACE_COUNT = 0x100
 BUILD_PREFER_FAST_TRACE = 0x1
+# size of solhip_trace_noalloc's fixed device array (SOLHIP_NOALLOC_SCRATCH_BYTES)
+NOALLOC_SCRATCH_BYTES = 128 << 20
 
 ERRORS = {0: "OK", -1: "INVALID_ARGUMENT", -2: "INVALID_HANDLE", -3: "OUT_OF_MEMORY", -4: "NO_DEVICE",
           -5: "DEVICE", -6: "UNSUPPORTED", -7: "OVERFLOW", -8: "STALE"}
@@ -117,6 +119,8 @@ SIGNATURES = {
     "solhip_program_create": (_i32, [_u64, _u32, C.POINTER(Spec), C.POINTER(_u64)]),
     "solhip_program_destroy": (_i32, [_u64, _u64]),
     "solhip_trace": (_i32, [_u64, _u64, _u64, _u64, C.POINTER(SceneTables), _vp, _i32, C.POINTER(Images), _u32, _u32]),
+    "solhip_trace_noalloc": (_i32, [_u64, _u64, _u64, _u64, C.POINTER(SceneTables), _vp, _i32, C.POINTER(Images), _u32,
+                                    _u32]),
     "solhip_stats": (_i32, [_u64, C.POINTER(Counters), _u32]),
     "solhip_profile": (_i32, [_u64, _u32]),
     "solhip_kernel_stats_get": (_i32, [_u64, _u32, C.POINTER(KernelStats), _u32]),
@@ -323,6 +327,13 @@ class Context:
         u = np.ascontiguousarray(uniforms, UNIFORMS)
         check(lib().solhip_trace(self.h, stream, prog, tlas, C.byref(tables) if tables is not None else None,
                                  _ptr(u), accum_start, C.byref(images), W, H))
+
+    def trace_noalloc(self, prog, tlas, tables, uniforms, accum_start, images, W, H, stream=0):
+        """solhip_trace with the AO scratch in the library's fixed device array
+        (no device allocation; one array per GPU, shared by all calls)"""
+        u = np.ascontiguousarray(uniforms, UNIFORMS)
+        check(lib().solhip_trace_noalloc(self.h, stream, prog, tlas, C.byref(tables) if tables is not None else None,
+                                         _ptr(u), accum_start, C.byref(images), W, H))
 
     def stats(self, reset=False):
         c = Counters()

@@ -48,6 +48,13 @@ def test_record_sizes_match_the_reference():
     assert api.Geometry.primitive_offset.offset == 32 and api.Geometry.index_count.offset == 40
 
 
+def test_noalloc_scratch_size_matches_the_header():
+    with open(os.path.join(ROOT, "include", "solhip.h")) as f:
+        m = re.search(r"#define SOLHIP_NOALLOC_SCRATCH_BYTES \((\d+)ull << (\d+)\)", f.read())
+    assert m and int(m.group(1)) << int(m.group(2)) == api.NOALLOC_SCRATCH_BYTES
+    assert 1920 * 1080 * (32 + 32) + 256 + 8192 <= api.NOALLOC_SCRATCH_BYTES
+
+
 def test_abi_version_and_spec_defaults():
     assert api.lib().solhip_abi_version() == 2
     assert api.ERRORS[-8] == "STALE"
