@@ -153,14 +153,17 @@ SOLHIP_HD V3 xform_dir(const float *m, V3 p)
 SOLHIP_HD float byte_f(uint32_t w, int b) { return (float)((w >> (8 * b)) & 0xFFu); }
 
 // loads through the global address space (the pointers come from memory, so
-// the compiler cannot infer it and would otherwise emit flat loads)
+// the compiler cannot infer it and would otherwise emit flat loads).  The
+// pointer goes through an integer: a direct cast from the generic pointer was
+// folded back into a flat load for the first element of a record, whose
+// s_waitcnt then also waited for the LDS stack operations.
 #if defined(__HIP_DEVICE_COMPILE__)
 typedef const __attribute__((address_space(1))) uint4 *gptr4;
 typedef const __attribute__((address_space(1))) uint2 *gptr2;
 typedef const __attribute__((address_space(1))) uint32_t *gptr1;
-#define SOLHIP_G4(p) ((gptr4)(p))
-#define SOLHIP_G2(p) ((gptr2)(p))
-#define SOLHIP_G1(p) ((gptr1)(p))
+#define SOLHIP_G4(p) ((gptr4)(uintptr_t)(p))
+#define SOLHIP_G2(p) ((gptr2)(uintptr_t)(p))
+#define SOLHIP_G1(p) ((gptr1)(uintptr_t)(p))
 #else
 #define SOLHIP_G4(p) ((const uint4 *)(p))
 #define SOLHIP_G2(p) ((const uint2 *)(p))

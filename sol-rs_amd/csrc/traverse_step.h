@@ -106,15 +106,20 @@ SOLHIP_HD bool trav_step(TravState &s, const SceneView &sv, uint32_t cull, uint2
             if (ANY) return true;
         }
     }
-    // 2. pops and BLAS exits up to the next node group or leaf group
+    // 2. pops and BLAS exits up to the next node group or leaf group.  An
+    // empty stack ends the ray at once, also inside a BLAS: nothing of the
+    // TLAS is pending (its work is pushed below the BLAS entry), so the world
+    // ray need not be restored.  Some lane of a wave finishes in almost every
+    // step, so the restore would otherwise be issued in almost every step.
     while (!s.tg_mask && !s.ng_hits) {
+        if (s.sp == 0) return true;
         if (s.level && s.sp == s.base) {
             // the BLAS is done: back to the world ray and the TLAS stack
             s.level = 0;
             s.o = s.wo;
             s.inv = safe_inv(s.wd);
             s.oct = octant(s.wd);
-        } else if (s.sp > 0) {
+        } else {
             --s.sp;
             const uint2 e = lds[s.sp * stride];
             if (e.y & kLeafGroupTag) {
@@ -125,8 +130,6 @@ SOLHIP_HD bool trav_step(TravState &s, const SceneView &sv, uint32_t cull, uint2
                 s.ng_hits = e.y & 0xFFu;
                 s.ng_imask = (e.y >> 8) & 0xFFu;
             }
-        } else {
-            return true;
         }
     }
     // 3. an instance leaf: enter its BLAS
