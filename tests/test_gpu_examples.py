@@ -118,3 +118,117 @@ def test_pathtrace_example_wavefront_nee_matches_oracle(oracle, tmp_path, stream
     gpu = np.fromfile(acc_file, np.float32).reshape(H, W, 4)
     assert np.count_nonzero(gpu.view(np.uint32) != ref.view(np.uint32)) == 0
     assert res["rays"] == rays
+
+
+def _write_tunnel_gltf(path):
+    """a two-primitive mesh (the shape of assets/models/tunnel.gltf, whose
+    mesh holds two primitives): an arched vault and a floor, with their own
+    materials, and a camera node looking down the tunnel; one data-URI
+    buffer.  SceneDescription::from_scene makes one BLAS per primitive
+    (src/ray/mod.rs:59-156)."""
+    import base64
+    segs, rings, L = 16, 12, 3.0
+    pos, nrm, idx = [], [], []
+    for r in range(rings + 1):
+        z = -L + 2 * L * r / rings
+        for s in range(segs + 1):
+            a = np.pi * s / segs
+            pos.append((np.cos(a), np.sin(a), z))
+            nrm.append((-np.cos(a), -np.sin(a), 0.0))
+    for r in range(rings):
+        for s in range(segs):
+            i0 = r * (segs + 1) + s
+            i1, i2, i3 = i0 + 1, i0 + segs + 1, i0 + segs + 2
+            idx += [i0, i2, i1, i1, i2, i3]
+    vault = (np.asarray(pos, np.float32), np.asarray(nrm, np.float32), np.asarray(idx, np.uint32))
+    fp = np.asarray([(-1, 0, -L), (1, 0, -L), (-1, 0, L), (1, 0, L)], np.float32)
+    floor = (fp, np.tile(np.asarray([0, 1, 0], np.float32), (4, 1)), np.asarray([0, 2, 1, 1, 2, 3], np.uint32))
+    blob, views, accs, prims = b"", [], [], []
+    for k, (p, n, ix) in enumerate((vault, floor)):
+        attrs = {}
+        for name, arr, typ in (("POSITION", p, "VEC3"), ("NORMAL", n, "VEC3"), (None, ix, "SCALAR")):
+            views.append({"buffer": 0, "byteOffset": len(blob), "byteLength": arr.nbytes})
+            acc = {"bufferView": len(views) - 1, "componentType": 5125 if name is None else 5126,
+                   "count": len(arr), "type": typ}
+            if name == "POSITION":
+                acc["min"] = [float(v) for v in p.min(0)]
+                acc["max"] = [float(v) for v in p.max(0)]
+            accs.append(acc)
+            blob += arr.tobytes()
+            if name:
+                attrs[name] = len(accs) - 1
+        prims.append({"attributes": attrs, "indices": len(accs) - 1, "material": k})
+    doc = {
+        "asset": {"version": "2.0"},
+        "buffers": [{"byteLength": len(blob),
+                     "uri": "data:application/octet-stream;base64," + base64.b64encode(blob).decode()}],
+        "bufferViews": views, "accessors": accs,
+        "materials": [
+            {"pbrMetallicRoughness": {"baseColorFactor": [0.8, 0.7, 0.6, 1.0], "metallicFactor": 0.0,
+                                      "roughnessFactor": 1.0}},
+            {"pbrMetallicRoughness": {"baseColorFactor": [0.3, 0.5, 0.8, 1.0], "metallicFactor": 0.5,
+                                      "roughnessFactor": 0.3}}],
+        "meshes": [{"name": "tunnel", "primitives": prims}],
+        "cameras": [{"type": "perspective", "perspective": {"aspectRatio": 1.33, "yfov": 45.0, "znear": 0.01,
+                                                           "zfar": 100.0}}],
+        # the node matrix is taken as the view matrix (src/scene/camera.rs:73-94)
+        "nodes": [{"mesh": 0}, {"camera": 0, "matrix": [-1, 0, 0, 0, 0, -1, 0, 0, 0, 0, 1, 0, 0, 0.5, -5.0, 1]}],
+        "scenes": [{"nodes": [0, 1]}], "scene": 0,
+    }
+    with open(path, "w") as f:
+        json.dump(doc, f)
+
+
+def test_tunnel_two_primitives_pathtrace_example_matches_oracle(oracle, tmp_path):
+    """the two-primitive mesh through load_scene -> from_scene (one BLAS and
+    one instance per primitive) -> the pathtrace.* program with the sky on"""
+    model = str(tmp_path / "tunnel.gltf")
+    _write_tunnel_gltf(model)
+    W, H, F = 64, 48, 2
+    d = str(tmp_path / "dump")
+    acc_file = str(tmp_path / "accum.f32")
+    res = _run("pathtrace", ["--model", model, "--size", "%dx%d" % (W, H), "--frames", str(F), "--dump", d,
+                             "--accum", acc_file, "--out", str(tmp_path / "pt.ppm"), "--sky"])
+    assert res["frames"] == F and res["overflow"] == 0
+    sd = _scene_from_dump(d)
+    assert len(sd.blas) == 2 and [b.tri_count for b in sd.blas] == [16 * 12 * 2, 2]
+    spec = solhip.spec_default(solhip.PROG_PATHTRACE_GLSL)
+    sc = oracle.Scene(sd)
+    ref = np.zeros((H, W, 4), np.float32)
+    ref8 = np.zeros((H, W, 4), np.uint8)
+    rays = 0
+    for f in range(F):
+        rays += sc.render_pathtrace_glsl(_uniforms(d, f), oracle.PTParams(1, spec.spp_per_frame, spec.max_depth),
+                                         0, ref, ref8, W, H)
+    gpu = np.fromfile(acc_file, np.float32).reshape(H, W, 4)
+    assert np.count_nonzero(gpu.view(np.uint32) != ref.view(np.uint32)) == 0
+    assert res["rays"] == rays
+    # both primitives are seen: the pixel-centre rays of the dumped camera hit both BLAS
+    from solhip.records import RAY
+    u = _uniforms(d, 0)
+    r = np.zeros(W * H, RAY)
+    for k in range(W * H):
+        r[k]["o"], r[k]["d"] = oracle.camera_ray(u, k % W, k // W, W, H, 0.5, 0.5)
+    r["tmin"], r["tmax"] = 1e-3, 1e4
+    hits = oracle.Scene(sd).trace(r)
+    assert set(np.unique(hits["inst"])) >= {0, 1}
+
+
+@pytest.mark.parametrize("model", ["cornell", "tunnel"])
+def test_ray_debug_example_matches_oracle(oracle, tmp_path, model):
+    """examples/3-ray-debug.rs over the mirror: debug.* program, the
+    example's camera (Camera::new(size), look_at((5,5,5), 0, -Y))"""
+    path = CORNELL
+    if model == "tunnel":
+        path = str(tmp_path / "tunnel.gltf")
+        _write_tunnel_gltf(path)
+    W, H = 50, 40
+    d = str(tmp_path / "dump")
+    tgt = str(tmp_path / "debug.f32")
+    res = _run("ray_debug", ["--model", path, "--size", "%dx%d" % (W, H), "--frames", "1", "--dump", d,
+                             "--target", tgt, "--out", str(tmp_path / "debug.ppm")])
+    assert res["frames"] == 1 and res["overflow"] == 0
+    sd = _scene_from_dump(d)
+    ref = oracle.Scene(sd).render_debug(_uniforms(d, 0), W, H)
+    gpu = np.fromfile(tgt, np.float32).reshape(H, W, 4)
+    assert np.count_nonzero(gpu.view(np.uint32) != ref.view(np.uint32)) == 0
