@@ -52,6 +52,7 @@ HBM_PEAK_GBS = 8000.0
 B_RAY_FIXED = 32 + 20
 B_NODE, B_TRI, B_INST = 80, 36, 64
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_ao_rays_heightfield1024_1080p.txt")
+C2_PMC_FILE = os.path.join(ROOT, "profiles", "pmc_pt_extend_instanced10k_1080p.txt")
 
 
 def parse_args():
@@ -101,15 +102,44 @@ def bytes_per_ray(c):
     return B_RAY_FIXED + (B_NODE * c.node_fetches + B_TRI * c.tri_fetches + B_INST * c.inst_fetches) / c.rays
 
 
+def read_profile(path):
+    """the 'key: number' lines of a committed rocprofv3 --pmc summary
+    (traffic_bytes_per_launch, l2_hit_rate, valu_wave_instructions_per_launch,
+    valu_lanes_active, valu_peak_wave_instr_per_s), {} without the file"""
+    try:
+        with open(path) as f:
+            text = f.read()
+    except OSError:
+        return {}
+    return {k: float(v) for k, v in re.findall(r"^([a-z0-9_]+):\s*([0-9.eE+-]+)\s*$", text, flags=re.M)}
+
+
 def read_traffic():
     """HBM bytes per launch of the AO-ray kernel from the committed rocprofv3
     --pmc summary (line 'traffic_bytes_per_launch: <int>'), else None."""
-    try:
-        with open(TRAFFIC_FILE) as f:
-            m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read())
-            return float(m.group(1)) if m else None
-    except OSError:
+    return read_profile(TRAFFIC_FILE).get("traffic_bytes_per_launch")
+
+
+def limiter(path, avg_launch_s):
+    """What binds the traversal kernel, from the counters of the committed
+    --pmc summary at path and this run's launch time: the VALU issue rate
+    against the fp32-FMA issue rate the probe sustains (the binding limit:
+    the kernels issue VALU instructions at most of that rate with about half
+    the lanes active), next to the measured HBM rate and the L2 hit rate."""
+    p = read_profile(path)
+    if "valu_wave_instructions_per_launch" not in p:
         return None
+    out = {"kind": "VALU issue", "source": os.path.relpath(path, ROOT)}
+    rate = p["valu_wave_instructions_per_launch"] / avg_launch_s
+    if "valu_peak_wave_instr_per_s" in p:
+        out["valu_issue_frac"] = round(rate / p["valu_peak_wave_instr_per_s"], 3)
+    out["valu_lanes_active"] = p.get("valu_lanes_active")
+    if "traffic_bytes_per_launch" in p:
+        gbs = p["traffic_bytes_per_launch"] / avg_launch_s / 1e9
+        out["hbm_measured_gbs"] = round(gbs, 1)
+        out["hbm_measured_frac"] = round(gbs / HBM_PEAK_GBS, 4)
+    out["l2_hit_rate"] = p.get("l2_hit_rate")
+    return out
 
 
 def main():
@@ -240,6 +270,9 @@ def main():
             },
             "avg_launch_ms": round(avg_launch_s * 1e3, 4),
             "rays_per_launch": int(rays_per_launch),
+            # the algorithmic HBM roofline above is not what binds this
+            # kernel: its node and triangle data come mostly from L2
+            "limiter": limiter(TRAFFIC_FILE, avg_launch_s) if dom_name == "ao_rays" else None,
         },
         "extra": {
             "rays_per_frame": int(rays / a.steps),
@@ -369,8 +402,11 @@ def extend_roofline(ctx, gs, sd, spec, img, W, H, rebuild, frame0):
     ext = ks["pt_extend"]
     b_ray = bytes_per_ray(cnt)
     achieved = b_ray * ext.counters.rays / (ext.total_ms / 1e3) / 1e9
+    avg_launch_s = ext.total_ms / max(1, ext.launches) / 1e3
     return {"bound": "hbm", "kernel": "pt_extend", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": read_profile(C2_PMC_FILE).get("traffic_bytes_per_launch"),
+            "limiter": limiter(C2_PMC_FILE, avg_launch_s),
             "bytes_per_ray": round(b_ray, 1),
             "fetches_per_ray": {"node": round(cnt.node_fetches / max(1, cnt.rays), 3),
                                 "tri": round(cnt.tri_fetches / max(1, cnt.rays), 3),
@@ -493,10 +529,12 @@ def bench_cache(a):
     ctx = solhip.Context(0)
     gs = solhip.GpuScene(ctx, sd)
 
-    def spec(nrc, spp, streams=0):
+    def spec(nrc, spp, streams=0, lr=0.0, steps=0):
         s = solhip.spec_default(solhip.PROG_PATHTRACE)
         s.enable_sky, s.spp_per_frame, s.max_depth, s.rr_start, s.nee, s.nrc = 2, spp, 8, 3, 1, nrc
         s.sample_streams = streams
+        s.nrc_learning_rate = lr      # 0: the library default, Adam lr 1e-2 (the configuration's)
+        s.nrc_train_steps = steps     # 0: 4 Adam steps per frame
         return s
 
     acc = ctx.upload(np.zeros((H, W, 4), np.float32))
@@ -510,10 +548,10 @@ def bench_cache(a):
     ctx.program_destroy(pr)
     norm = float(np.sqrt((ref ** 2).mean()))
 
-    def run(nrc, frames, checkpoints, warm=0):
+    def run(nrc, frames, checkpoints, warm=0, lr=0.0, steps=0):
         """frames of 1 spp into a running mean; with warm > 0 the first warm
         frames train the cache without being kept (their time counts)"""
-        p = ctx.program(solhip.PROG_PATHTRACE, spec(nrc, 1))
+        p = ctx.program(solhip.PROG_PATHTRACE, spec(nrc, 1, lr=lr, steps=steps))
         out = []
         t = 0.0
         for f in range(warm + frames):
@@ -531,11 +569,29 @@ def bench_cache(a):
         ctx.program_destroy(p)
         return out
 
+    def floor(curve):
+        """least-squares fit of rel_rmse^2 = bias^2 + var / spp over the
+        checkpoints from 16 spp on: the error a run keeps however long it
+        runs (bias) and the per-sample variance"""
+        pts = [(c["spp"], c["rel_rmse"] ** 2) for c in curve if c["spp"] >= 16]
+        A = np.array([[1.0, 1.0 / n] for n, _ in pts])
+        y = np.array([e for _, e in pts])
+        (b2, v), *_ = np.linalg.lstsq(A, y, rcond=None)
+        return {"bias": round(float(np.sqrt(max(b2, 0.0))), 5), "sigma_1spp": round(float(np.sqrt(max(v, 0.0))), 4)}
+
     cps = [1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024]
     plain = run(0, 1024, cps)
     cache = run(1, 1024, cps, warm=64)
     target = plain[-1]["rel_rmse"]
     reach = next((c for c in cache if c["rel_rmse"] <= target), None)
+    # experiments on the cache's error floor: the step size, and training
+    # four times as long per frame
+    experiments = []
+    for name, lr, steps in (("adam lr 1e-3", 1e-3, 0), ("16 Adam steps per frame", 0.0, 16)):
+        print("cache-c4: experiment %s" % name, file=sys.stderr, flush=True)
+        c = run(1, 1024, cps, warm=64, lr=lr, steps=steps)
+        experiments.append({"name": name, "curve": c, "fit": floor(c),
+                            "rel_rmse_1024": c[-1]["rel_rmse"], "seconds_1024": c[-1]["seconds"]})
     result = {
         "metric": "learned-cache time to the plain tracer's 1024-spp error, fraction of the plain time",
         "value": round(reach["seconds"] / plain[-1]["seconds"], 4) if reach else None,
@@ -544,11 +600,12 @@ def bench_cache(a):
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (network) / f32",
         "data": "synthetic",
         "config": {"workload": "10000 instances of 5 BLAS, %dx%d, 1 spp per frame, depth 8, RR from 3, NEE, "
-                               "gradient sky; cache: 64-64x4-3 ReLU MLP, BF16 MFMA, 4 Adam steps per frame on "
+                               "gradient sky; cache: 64-64x4-3 ReLU MLP, BF16 MFMA, 4 Adam steps (lr 1e-2) per frame on "
                                "<= 65536 rows from 1-in-R training paths, 64 warm-up frames (timed, not "
                                "accumulated); reference 4096 spp plain" % (W, H),
                    "parallelism": "single GPU"},
-        "extra": {"plain": plain, "cache": cache, "plain_1024spp_rel_rmse": target},
+        "extra": {"plain": plain, "cache": cache, "plain_1024spp_rel_rmse": target,
+                  "fit": {"plain": floor(plain), "cache": floor(cache)}, "experiments": experiments},
     }
     ctx.close()
     print(json.dumps(result))

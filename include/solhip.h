@@ -187,7 +187,7 @@ typedef struct {
      * (training reads its batch size back). */
     uint32_t nrc;             /* 1: use the cache */
     uint32_t nrc_train_steps; /* Adam steps per frame (0 = 4) */
-    float nrc_learning_rate;  /* Adam step size (0 = 1e-3) */
+    float nrc_learning_rate;  /* Adam step size (0 = 1e-2) */
     /* SOLHIP_PROG_PATHTRACE: the image's 8x8 tiles split into this many
      * contiguous slices whose passes run on their own streams at the same
      * time, so one slice's slowest rays overlap the others' work; pixels are

@@ -1,7 +1,7 @@
 """Build libsolhip.so (HIP, gfx950) in-tree.
 
     python sol-rs_amd/build.py            # compile every csrc/*.hip, link lib/libsolhip.so,
-                                          # then the host mirror and examples
+                                          # then the host mirror, examples and bin/valu_peak
 
 Objects go to sol-rs_amd/_build/, the libraries to sol-rs_amd/lib/ and the
 example programs to sol-rs_amd/bin/.  hipcc cross-compiles for gfx950 without
@@ -101,6 +101,20 @@ def build_host(verbose=True):
     return HOST_LIB
 
 
+def build_measure(verbose=True):
+    """bin/valu_peak: the fp32 VALU issue-rate probe (measure/valu_peak.hip)
+    whose figure the traversal kernels' counters are compared with"""
+    os.makedirs(BINDIR, exist_ok=True)
+    src = os.path.join(HERE, "measure", "valu_peak.hip")
+    exe = os.path.join(BINDIR, "valu_peak")
+    _run([HIPCC] + HIP_FLAGS + ["-o", exe + ".tmp", src])
+    os.replace(exe + ".tmp", exe)
+    if verbose:
+        print("built", exe)
+    return exe
+
+
 if __name__ == "__main__":
     build()
     build_host()
+    build_measure()

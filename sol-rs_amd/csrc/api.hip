@@ -294,7 +294,7 @@ static int nrc_query_hook(void *user, hipStream_t s)
 int trace_nrc(Context *c, hipStream_t s, Program *p, Tlas *t, WfArgs &w, Timer *timer)
 {
     if (!p->nrc_mlp) {
-        const float lr = p->spec.nrc_learning_rate > 0.0f ? p->spec.nrc_learning_rate : 1e-3f;
+        const float lr = p->spec.nrc_learning_rate > 0.0f ? p->spec.nrc_learning_rate : 1e-2f;
         p->nrc_mlp = mlp_new(c, 1u, lr);
         if (!p->nrc_mlp) return SOLHIP_ERROR_DEVICE;
     }
