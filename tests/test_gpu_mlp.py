@@ -135,9 +135,9 @@ def test_adam_step_and_training(ctx):
     m.close()
 
 
-def _nrc_frames(ctx, gs, sd, W, H, frames, nrc, spp=1, depth=6):
+def _nrc_frames(ctx, gs, sd, W, H, frames, nrc, spp=1, depth=6, sky=0, rr=0):
     s = solhip.spec_default(solhip.PROG_PATHTRACE)
-    s.enable_sky, s.spp_per_frame, s.max_depth, s.nee, s.nrc = 0, spp, depth, 1, nrc
+    s.enable_sky, s.spp_per_frame, s.max_depth, s.nee, s.nrc, s.rr_start = sky, spp, depth, 1, nrc, rr
     prog = ctx.program(solhip.PROG_PATHTRACE, s)
     acc = ctx.upload(np.zeros((H, W, 4), np.float32))
     img = solhip.Images(acc.h, 0, 0, 0, 0, 0)
@@ -168,3 +168,39 @@ def test_radiance_cache_learns_the_cornell_box(ctx):
     print("relative RMSE by frame:", [round(r, 4) for r in rel[::8]], flush=True)
     assert rel[-1] < 0.2, rel[-1]
     assert rel[-1] < 0.7 * rel[1], (rel[1], rel[-1])
+
+
+def test_radiance_cache_on_the_instanced_scene(ctx, oracle):
+    """the cache on the batched instanced workload (its generator at 200
+    instances of the five BLAS kinds; gradient sky, depth 8, roulette from
+    bounce 3, next-event estimation, paths ended at their first bounce vertex
+    into the cache): its running mean approaches the converged plain image
+    and far below an untrained cache's first frames; a plain program on the
+    same scene afterwards still renders the oracle's image bit for bit"""
+    from solhip import scenes
+    sd = scenes.instanced(200, seed=2, tube=(50, 100), spread=8.0)
+    gs = solhip.GpuScene(ctx, sd)
+    W, H = 64, 48
+    print("reference (plain, 1024 spp)", flush=True)
+    ref = _nrc_frames(ctx, gs, sd, W, H, 4, 0, spp=256, depth=8, sky=2, rr=3)[-1]
+    print("cache frames", flush=True)
+    imgs = _nrc_frames(ctx, gs, sd, W, H, 64, 1, depth=8, sky=2, rr=3)
+    for im in imgs:
+        assert np.isfinite(im).all() and (im >= 0).all()
+    norm = np.sqrt((ref ** 2).mean())
+    rel = [np.sqrt(((im - ref) ** 2).mean()) / norm for im in imgs]
+    print("relative RMSE by frame:", [round(r, 4) for r in rel[::8]], flush=True)
+    assert rel[-1] < 0.25, rel[-1]
+    assert rel[-1] < 0.7 * rel[1], (rel[1], rel[-1])
+    # nrc = 0 after the cache ran: the plain estimator, bit-identical with the oracle
+    s = solhip.spec_default(solhip.PROG_PATHTRACE)
+    s.enable_sky, s.spp_per_frame, s.max_depth, s.nee, s.rr_start = 2, 2, 8, 1, 3
+    prog = ctx.program(solhip.PROG_PATHTRACE, s)
+    acc = ctx.upload(np.zeros((H, W, 4), np.float32))
+    u = sd.uniforms(W, H, 5)
+    ctx.trace(prog, gs.tlas, gs.tables, u, 5, solhip.Images(acc.h, 0, 0, 0, 0, 0), W, H)
+    got = acc.download(np.float32).reshape(H, W, 4)
+    want = np.zeros((H, W, 4), np.float32)
+    oracle.Scene(sd).render_pathtrace(u, oracle.PTConfig(2, 2, 8, 3, 1, 0), 5, want, None, W, H)
+    assert np.count_nonzero(got.view(np.uint32) != want.view(np.uint32)) == 0
+    ctx.program_destroy(prog)
