@@ -211,7 +211,7 @@ __device__ __forceinline__ void srv_trace(const SrvArgs &A, const QRay *q, uint3
             }
         }
         if (__ballot(has) == 0 && pf.exhausted) break;
-        if (has && trav_step<ANY, false>(st, sv, 0xFFu, stack + threadIdx.x, kSrvBlock, tc)) {
+        if (has && trav_step<ANY, false, false>(st, sv, 0xFFu, stack + threadIdx.x, kSrvBlock, tc)) {
             const Hit h = trav_result(st);
             ++rays;
             if (!ANY) {

@@ -241,7 +241,7 @@ __device__ __forceinline__ void trace_queue(const WfArgs &A, const QRay *q, uint
             }
         }
         if (__ballot(has) == 0 && pf.exhausted) break;
-        if (has && trav_step<ANY, COUNT>(st, A.sv, 0xFFu, stack + threadIdx.x, A.block, tc)) {
+        if (has && trav_step<ANY, COUNT, false>(st, A.sv, 0xFFu, stack + threadIdx.x, A.block, tc)) {
             queue_result<ANY>(A, idx, path, trav_result(st));
             ++rays;
             has = false;
