@@ -88,7 +88,8 @@ SOLHIP_HD Hit trav_result(const TravState &s)
 // state (three more registers) instead of recomputing it (three correctly
 // rounded divisions).  Measured per kernel: the AO kernels run 6 % faster
 // with it; the path tracer's extension kernel, whose rays leave a BLAS 3.9
-// times on average, 1 % slower (the extra registers spill there).
+// times on average, 6 % faster once its register budget holds the three
+// registers (at 5 waves per SIMD they spilled and it ran 1 % slower).
 template <bool ANY, bool COUNT, bool WINV = true>
 SOLHIP_HD bool trav_step(TravState &s, const SceneView &sv, uint32_t cull, uint2 *lds, int stride, TravCounts &cnt)
 {

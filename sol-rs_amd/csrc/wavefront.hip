@@ -161,9 +161,17 @@ __global__ void k_pt_fold(WfArgs A, uint32_t pass)
     A.fsum[pix] = fs;
 }
 
-// register budget of the traversal kernels: waves per SIMD they must fit
+// Register budget of the traversal kernels (waves per SIMD they must fit).
+// BLAS exits restore the world ray's inverse direction from the ray state
+// (trav_step's WINV), and the traversal kernels ask for 4 waves per SIMD so
+// that those registers fit: on 10,000 instances (3.9 BLAS exits per ray)
+// 1279 -> 1356 Mrays/s; the inverse at 5 waves spilled (1275), 4 waves
+// without it gave 1295
+#ifndef SOLHIP_WF_WINV
+#define SOLHIP_WF_WINV true
+#endif
 #ifndef SOLHIP_WF_WAVES
-#define SOLHIP_WF_WAVES 5
+#define SOLHIP_WF_WAVES 4
 #endif
 #define SOLHIP_WF_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SOLHIP_WF_WAVES)))
 
@@ -241,7 +249,7 @@ __device__ __forceinline__ void trace_queue(const WfArgs &A, const QRay *q, uint
             }
         }
         if (__ballot(has) == 0 && pf.exhausted) break;
-        if (has && trav_step<ANY, COUNT, false>(st, A.sv, 0xFFu, stack + threadIdx.x, A.block, tc)) {
+        if (has && trav_step<ANY, COUNT, SOLHIP_WF_WINV>(st, A.sv, 0xFFu, stack + threadIdx.x, A.block, tc)) {
             queue_result<ANY>(A, idx, path, trav_result(st));
             ++rays;
             has = false;
