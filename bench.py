@@ -382,8 +382,14 @@ def extend_roofline(ctx, gs, sd, spec, img, W, H, rebuild, frame0):
     launch durations from hipEvents on the launch stream over two more
     frames.  Events are per launch, so these frames run eagerly on one image
     slice (the captured graph and the slices of the timed region would
-    hide the launches); the kernel and its launch shape are the same."""
+    hide the launches), on the library's default persistent grid (the
+    launch shape of the committed counter summary); the kernel is the same."""
     import solhip
+    # the library's default persistent grid, which fills the GPU on its own:
+    # the autotuned shape is chosen for image slices running side by side and
+    # leaves the GPU half empty in a one-slice launch
+    spec = type(spec).from_buffer_copy(spec)
+    spec.wf_block, spec.wf_blocks_per_cu = 0, 0
     rebuild()
     ctx.profile(solhip.api.PROFILE_COUNT)
     for k in ("pt_extend", "pt_shade", "pt_shadow"):
@@ -414,7 +420,7 @@ def extend_roofline(ctx, gs, sd, spec, img, W, H, rebuild, frame0):
             "avg_launch_ms": round(ext.total_ms / max(1, ext.launches), 4),
             "rays_per_launch": int(ext.counters.rays / max(1, ext.launches)),
             "kernel_ms_per_frame": {k: round(v.total_ms / 2, 3) for k, v in ks.items()},
-            "timing": "hipEvents per launch over 2 eager one-slice frames after the timed region"}
+            "timing": "hipEvents per launch over 2 eager one-slice frames after the timed region, default persistent grid"}
 
 
 _progs = {}
