@@ -41,8 +41,14 @@ namespace {
 constexpr uint32_t kSrvBlock = 64;
 
 // register budget of the traversal kernels: waves per SIMD they must fit
+// BLAS exits restore the world ray's inverse direction from the ray state
+// (trav_step's WINV), with a budget of 4 waves per SIMD for its registers:
+// C3 capacity 12.04 -> 12.09 jobs/s, p50 at half load 130 -> 122 ms
+#ifndef SOLHIP_SRV_WINV
+#define SOLHIP_SRV_WINV true
+#endif
 #ifndef SOLHIP_SRV_WAVES
-#define SOLHIP_SRV_WAVES 5
+#define SOLHIP_SRV_WAVES 4
 #endif
 #define SOLHIP_SRV_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SOLHIP_SRV_WAVES)))
 // fetch partitions of the extend and shadow kernels (fetch.h)
@@ -211,7 +217,7 @@ __device__ __forceinline__ void srv_trace(const SrvArgs &A, const QRay *q, uint3
             }
         }
         if (__ballot(has) == 0 && pf.exhausted) break;
-        if (has && trav_step<ANY, false, false>(st, sv, 0xFFu, stack + threadIdx.x, kSrvBlock, tc)) {
+        if (has && trav_step<ANY, false, SOLHIP_SRV_WINV>(st, sv, 0xFFu, stack + threadIdx.x, kSrvBlock, tc)) {
             const Hit h = trav_result(st);
             ++rays;
             if (!ANY) {
